@@ -1,0 +1,1316 @@
+// MI355X (gfx950) executor of the directional FMM evaluation pass and its C
+// ABI (include/defmm_amd.h).  One evaluation = gather charges into tree order,
+// P2M, M2M per level, M2F, fused M2L+F2L, L2L per level, L2P, P2P near field,
+// scatter -- the sequence of gemv (reference include/interface.hpp:153-170)
+// with every tree walk replaced by a flat list built at plan time.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/defmm_amd.h"
+#include "kernels.hpp"
+#include "plan.hpp"
+
+namespace dfm {
+
+static thread_local std::string g_err;
+
+// ----------------------------------------------------------------------------
+// kernels
+// ----------------------------------------------------------------------------
+
+// fill a complex array with zeros (used instead of hipMemsetAsync so that the
+// sequence can be captured in a graph)
+template <typename R2>
+__global__ void k_zero(R2* a, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    R2 z;
+    z.x = 0;
+    z.y = 0;
+    a[i] = z;
+  }
+}
+
+// Tree::loadRHS (tree.hpp:149-154): q_tree[i] = q_in[perm[i]]
+template <typename R>
+__global__ void k_gather(const double2* __restrict__ qin, const int* __restrict__ perm,
+                         typename C2T<R>::T* __restrict__ q, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    double2 v = qin[perm[i]];
+    q[i] = c2<R>((R)v.x, (R)v.y);
+  }
+}
+
+// Tree::writeLHS (tree.hpp:156-160): p_out[perm[i]] = p_tree[i]
+template <typename R>
+__global__ void k_scatter(const typename C2T<R>::T* __restrict__ p, const int* __restrict__ perm,
+                          double2* __restrict__ pout, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    typename C2T<R>::T v = p[i];
+    double2 o;
+    o.x = (double)v.x;
+    o.y = (double)v.y;
+    pout[perm[i]] = o;
+  }
+}
+
+// P2M (P2M.hpp:113-136 with the leaf matrices of mapOnTrees.hpp:36-83):
+//   HF leaf, slot of direction d:
+//     M[I] = conj(sig2_d[I]) * sum_j S_I((y_j-c)/r) exp(-i kappa d.(y_j-c)) q_j
+//   LF leaf: M[I] = sum_j S_I((y_j-c)/r) q_j
+// One block per leaf; particles are staged 64 at a time in LDS.
+template <typename R, int L>
+__global__ __launch_bounds__(128) void k_p2m(const int* __restrict__ leaves, DevCells S,
+                                             DevLevels lv, const double* __restrict__ px,
+                                             const double* __restrict__ py,
+                                             const double* __restrict__ pz,
+                                             const typename C2T<R>::T* __restrict__ q,
+                                             const double* __restrict__ dirs,
+                                             const typename C2T<R>::T* __restrict__ sig2,
+                                             OpConsts<R> oc, double kappa,
+                                             typename C2T<R>::T* __restrict__ M) {
+  typedef typename C2T<R>::T R2;
+  constexpr int LL = L * L * L;
+  constexpr int NI = (LL + 127) / 128;
+  __shared__ R Sw[3][L][64];
+  __shared__ R2 w[64];
+  __shared__ double rel[3][64];
+  const int c = leaves[blockIdx.x];
+  const int lvl = S.lvl[c], pb = S.pbeg[c], n = S.nprt[c], s0 = S.slot0[c], ns = S.nslot[c];
+  const double cx = S.ctr[3 * c], cy = S.ctr[3 * c + 1], cz = S.ctr[3 * c + 2], rad = S.rad[c];
+  const int hf = lv.isHF[lvl];
+  const int doff = lv.dirOff[lvl], srow = lv.sigRow0[lvl];
+  const bool single = n <= 64;
+  const int tid = threadIdx.x;
+  for (int k = 0; k < ns; k++) {
+    const int d = hf ? S.slotDir[s0 + k] : 0;
+    double dx = 0., dy = 0., dz = 0.;
+    if (hf) {
+      dx = dirs[3 * (doff + d)];
+      dy = dirs[3 * (doff + d) + 1];
+      dz = dirs[3 * (doff + d) + 2];
+    }
+    R2 acc[NI];
+#pragma unroll
+    for (int m = 0; m < NI; m++) acc[m] = c2<R>(0, 0);
+    for (int j0 = 0; j0 < n; j0 += 64) {
+      const int cn = min(64, n - j0);
+      __syncthreads();
+      if (tid < cn) {
+        const int j = tid;
+        if (!single || k == 0) {
+          double r0 = px[pb + j0 + j] - cx, r1 = py[pb + j0 + j] - cy, r2 = pz[pb + j0 + j] - cz;
+          rel[0][j] = r0;
+          rel[1][j] = r1;
+          rel[2][j] = r2;
+          R t[LMAX];
+          lagrangeAll<R>((R)(r0 / rad), L, oc.xi, oc.iden, t);
+#pragma unroll
+          for (int i = 0; i < L; i++) Sw[0][i][j] = t[i];
+          lagrangeAll<R>((R)(r1 / rad), L, oc.xi, oc.iden, t);
+#pragma unroll
+          for (int i = 0; i < L; i++) Sw[1][i][j] = t[i];
+          lagrangeAll<R>((R)(r2 / rad), L, oc.xi, oc.iden, t);
+#pragma unroll
+          for (int i = 0; i < L; i++) Sw[2][i][j] = t[i];
+        }
+        R2 qq = q[pb + j0 + j];
+        if (hf) {
+          R cs, sn;
+          expi(-kappa * (dx * rel[0][j] + dy * rel[1][j] + dz * rel[2][j]), cs, sn);
+          w[j] = cmul(qq, c2<R>(cs, sn));
+        } else {
+          w[j] = qq;
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < NI; m++) {
+        const int I = tid + 128 * m;
+        if (I < LL) {
+          const int i0 = I / (L * L), i1 = (I / L) % L, i2 = I % L;
+          R2 a = acc[m];
+          for (int j = 0; j < cn; j++) {
+            R s = Sw[0][i0][j] * Sw[1][i1][j] * Sw[2][i2][j];
+            a.x += s * w[j].x;
+            a.y += s * w[j].y;
+          }
+          acc[m] = a;
+        }
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < NI; m++) {
+      const int I = tid + 128 * m;
+      if (I < LL) {
+        R2 v = acc[m];
+        if (hf) v = cmulc(v, sig2[(size_t)(srow + d) * LL + I]);
+        M[(size_t)(s0 + k) * LL + I] = v;
+      }
+    }
+  }
+}
+
+// Separable translation of one level (M2M: M2M.hpp:34-124 with the sweeps of
+// interpolation.hpp:244-361; L2L: L2L.hpp:34-127 with interpolation.hpp:
+// 395-513).  For every output slot o:
+//   out[o] (+)= sum_e edv_e * conj(post_e) .* Sweep_{sgn_e}(pre_e .* in[e])
+// where Sweep applies the L x L matrix mat[(sgn >> k) & 1] along axis k
+// (axis 2 first, as the reference's three GEMM + rotation passes do), and
+// pre/post are rows of the phase tables (absent for LF levels).
+template <typename R, int L>
+__global__ __launch_bounds__(128) void k_trans(
+    const int* __restrict__ out, const int* __restrict__ beg, const int* __restrict__ in,
+    const int* __restrict__ sgn, const int* __restrict__ sigIdx, const double2* __restrict__ edv,
+    const typename C2T<R>::T* __restrict__ src, typename C2T<R>::T* __restrict__ dst,
+    const typename C2T<R>::T* __restrict__ preTab, const typename C2T<R>::T* __restrict__ postTab,
+    const R* __restrict__ mat0, const R* __restrict__ mat1, int accumulate) {
+  typedef typename C2T<R>::T R2;
+  constexpr int LL = L * L * L;
+  constexpr int NI = (LL + 127) / 128;
+  __shared__ R2 a[LL], b[LL];
+  __shared__ R mt[2][L * L];
+  const int tid = threadIdx.x;
+  for (int x = tid; x < L * L; x += 128) {
+    mt[0][x] = mat0[x];
+    mt[1][x] = mat1[x];
+  }
+  const int o = blockIdx.x;
+  const int e0 = beg[o], e1 = beg[o + 1];
+  R2 acc[NI];
+#pragma unroll
+  for (int m = 0; m < NI; m++) acc[m] = c2<R>(0, 0);
+  for (int e = e0; e < e1; e++) {
+    const int s = in[e], row = sigIdx[e], sg = sgn[e];
+    const double2 ed = edv[e];
+    const R2 edr = c2<R>((R)ed.x, (R)ed.y);
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < NI; m++) {
+      const int I = tid + 128 * m;
+      if (I < LL) {
+        R2 v = src[(size_t)s * LL + I];
+        if (row >= 0) v = cmul(v, preTab[(size_t)row * LL + I]);
+        a[I] = v;
+      }
+    }
+    __syncthreads();
+    {
+      const R* M2 = mt[(sg >> 2) & 1];
+#pragma unroll
+      for (int m = 0; m < NI; m++) {
+        const int I = tid + 128 * m;
+        if (I < LL) {
+          const int i2 = I % L, base = I - i2;
+          R2 t = c2<R>(0, 0);
+#pragma unroll
+          for (int j = 0; j < L; j++) {
+            t.x += M2[i2 * L + j] * a[base + j].x;
+            t.y += M2[i2 * L + j] * a[base + j].y;
+          }
+          b[I] = t;
+        }
+      }
+    }
+    __syncthreads();
+    {
+      const R* M1 = mt[(sg >> 1) & 1];
+#pragma unroll
+      for (int m = 0; m < NI; m++) {
+        const int I = tid + 128 * m;
+        if (I < LL) {
+          const int i1 = (I / L) % L, base = I - i1 * L;
+          R2 t = c2<R>(0, 0);
+#pragma unroll
+          for (int j = 0; j < L; j++) {
+            t.x += M1[i1 * L + j] * b[base + j * L].x;
+            t.y += M1[i1 * L + j] * b[base + j * L].y;
+          }
+          a[I] = t;
+        }
+      }
+    }
+    __syncthreads();
+    {
+      const R* M0 = mt[sg & 1];
+#pragma unroll
+      for (int m = 0; m < NI; m++) {
+        const int I = tid + 128 * m;
+        if (I < LL) {
+          const int i0 = I / (L * L), base = I - i0 * L * L;
+          R2 t = c2<R>(0, 0);
+#pragma unroll
+          for (int j = 0; j < L; j++) {
+            t.x += M0[i0 * L + j] * a[base + j * L * L].x;
+            t.y += M0[i0 * L + j] * a[base + j * L * L].y;
+          }
+          if (row >= 0) t = cmulc(t, postTab[(size_t)row * LL + I]);
+          t = cmul(t, edr);
+          acc[m].x += t.x;
+          acc[m].y += t.y;
+        }
+      }
+    }
+  }
+  const int os = out[o];
+#pragma unroll
+  for (int m = 0; m < NI; m++) {
+    const int I = tid + 128 * m;
+    if (I < LL) {
+      R2 v = acc[m];
+      if (accumulate) {
+        R2 old = dst[(size_t)os * LL + I];
+        v.x += old.x;
+        v.y += old.y;
+      }
+      dst[(size_t)os * LL + I] = v;
+    }
+  }
+}
+
+// M2F (M2F.hpp:35-46, circulant.hpp:95-106): F = DFT_P^3 of M embedded in
+// the (2L-1)^3 grid at I2i(I, P), unnormalised, exp(-2 pi i jk/P).  The
+// embedding is zero outside the L^3 corner, so the three axis passes sum over
+// L inputs only.
+template <typename R, int L>
+__global__ __launch_bounds__(256) void k_m2f(const int* __restrict__ m2fSlot,
+                                             const typename C2T<R>::T* __restrict__ M,
+                                             typename C2T<R>::T* __restrict__ F,
+                                             const R* __restrict__ twc,
+                                             const R* __restrict__ tws) {
+  typedef typename C2T<R>::T R2;
+  constexpr int P = 2 * L - 1, LL = L * L * L, PP = P * P * P;
+  __shared__ R2 sM[LL], T1[L * L * P], T2[L * P * P];
+  __shared__ R wc[P], ws[P];
+  const int tid = threadIdx.x;
+  if (tid < P) {
+    wc[tid] = twc[tid];
+    ws[tid] = tws[tid];
+  }
+  const int s = m2fSlot[blockIdx.x];
+  for (int I = tid; I < LL; I += 256) sM[I] = M[(size_t)s * LL + I];
+  __syncthreads();
+  for (int x = tid; x < L * L * P; x += 256) {
+    const int k2 = x % P, r = x / P;
+    R2 t = c2<R>(0, 0);
+#pragma unroll
+    for (int j = 0; j < L; j++) {
+      const int m = (j * k2) % P;
+      const R2 v = sM[r * L + j];
+      t.x += v.x * wc[m] + v.y * ws[m];
+      t.y += v.y * wc[m] - v.x * ws[m];
+    }
+    T1[x] = t;
+  }
+  __syncthreads();
+  for (int x = tid; x < L * P * P; x += 256) {
+    const int k2 = x % P, k1 = (x / P) % P, i0 = x / (P * P);
+    R2 t = c2<R>(0, 0);
+#pragma unroll
+    for (int j = 0; j < L; j++) {
+      const int m = (j * k1) % P;
+      const R2 v = T1[(i0 * L + j) * P + k2];
+      t.x += v.x * wc[m] + v.y * ws[m];
+      t.y += v.y * wc[m] - v.x * ws[m];
+    }
+    T2[x] = t;
+  }
+  __syncthreads();
+  R2* Fr = F + (size_t)blockIdx.x * PP;
+  for (int x = tid; x < PP; x += 256) {
+    const int k0 = x / (P * P), r = x % (P * P);
+    R2 t = c2<R>(0, 0);
+#pragma unroll
+    for (int j = 0; j < L; j++) {
+      const int m = (j * k0) % P;
+      const R2 v = T2[j * P * P + r];
+      t.x += v.x * wc[m] + v.y * ws[m];
+      t.y += v.y * wc[m] - v.x * ws[m];
+    }
+    Fr[x] = t;
+  }
+}
+
+// Fused M2L + F2L for one target slot (M2L.hpp:114-157, F2L.hpp:35-46,
+// circulant.hpp:108-118):
+//   J[w] = sum_e U_{tab_e}[perm_{h_e}[w]] * F_{src_e}[w]
+//   L[I] = (1/PP) * IDFT(J)[I2i(I, P)]     (written, not accumulated)
+// J never leaves LDS; the inverse transform only produces the L^3 corner.
+template <typename R, int L>
+__global__ __launch_bounds__(256) void k_m2l(const int* __restrict__ beg,
+                                             const int* __restrict__ srcF,
+                                             const int* __restrict__ tabHash,
+                                             const typename C2T<R>::T* __restrict__ F,
+                                             const typename C2T<R>::T* __restrict__ U,
+                                             const int* __restrict__ perm,
+                                             typename C2T<R>::T* __restrict__ Lout,
+                                             const R* __restrict__ twc,
+                                             const R* __restrict__ tws) {
+  typedef typename C2T<R>::T R2;
+  constexpr int P = 2 * L - 1, LL = L * L * L, PP = P * P * P;
+  constexpr int NW = (PP + 255) / 256;
+  __shared__ R2 J[PP], T1[L * P * P], T2[L * L * P];
+  __shared__ R wc[P], ws[P];
+  const int tid = threadIdx.x;
+  const int t = blockIdx.x;
+  const int e0 = beg[t], e1 = beg[t + 1];
+  R2* Lr = Lout + (size_t)t * LL;
+  if (e0 == e1) {
+    for (int I = tid; I < LL; I += 256) Lr[I] = c2<R>(0, 0);
+    return;
+  }
+  if (tid < P) {
+    wc[tid] = twc[tid];
+    ws[tid] = tws[tid];
+  }
+  R2 acc[NW];
+#pragma unroll
+  for (int m = 0; m < NW; m++) acc[m] = c2<R>(0, 0);
+  for (int e = e0; e < e1; e++) {
+    const int th = tabHash[e];
+    const R2* Fr = F + (size_t)srcF[e] * PP;
+    const R2* Ur = U + (size_t)(th >> 6) * PP;
+    const int* pr = perm + (th & 63) * PP;
+#pragma unroll
+    for (int m = 0; m < NW; m++) {
+      const int w = tid + 256 * m;
+      if (w < PP) {
+        const R2 u = Ur[pr[w]];
+        const R2 f = Fr[w];
+        acc[m].x += u.x * f.x - u.y * f.y;
+        acc[m].y += u.x * f.y + u.y * f.x;
+      }
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < NW; m++) {
+    const int w = tid + 256 * m;
+    if (w < PP) J[w] = acc[m];
+  }
+  __syncthreads();
+  // axis 0: T1[(k0*P + w1)*P + w2], k0 < L
+  for (int x = tid; x < L * P * P; x += 256) {
+    const int r = x % (P * P), k0 = x / (P * P);
+    R2 s = c2<R>(0, 0);
+#pragma unroll
+    for (int w0 = 0; w0 < P; w0++) {
+      const int m = (w0 * k0) % P;
+      const R2 v = J[w0 * P * P + r];
+      s.x += v.x * wc[m] - v.y * ws[m];
+      s.y += v.y * wc[m] + v.x * ws[m];
+    }
+    T1[x] = s;
+  }
+  __syncthreads();
+  // axis 1: T2[(k0*L + k1)*P + w2], k1 < L
+  for (int x = tid; x < L * L * P; x += 256) {
+    const int w2 = x % P, k1 = (x / P) % L, k0 = x / (L * P);
+    R2 s = c2<R>(0, 0);
+#pragma unroll
+    for (int w1 = 0; w1 < P; w1++) {
+      const int m = (w1 * k1) % P;
+      const R2 v = T1[(k0 * P + w1) * P + w2];
+      s.x += v.x * wc[m] - v.y * ws[m];
+      s.y += v.y * wc[m] + v.x * ws[m];
+    }
+    T2[x] = s;
+  }
+  __syncthreads();
+  // axis 2 and scaling: L[(k0*L + k1)*L + k2]
+  const R inv = (R)(1.0 / (double)PP);
+  for (int x = tid; x < LL; x += 256) {
+    const int k2 = x % L, r = x / L;
+    R2 s = c2<R>(0, 0);
+#pragma unroll
+    for (int w2 = 0; w2 < P; w2++) {
+      const int m = (w2 * k2) % P;
+      const R2 v = T2[r * P + w2];
+      s.x += v.x * wc[m] - v.y * ws[m];
+      s.y += v.y * wc[m] + v.x * ws[m];
+    }
+    Lr[x] = c2<R>(s.x * inv, s.y * inv);
+  }
+}
+
+// L2P (L2P.hpp:121-173 with mapOnTrees.hpp:85-134):
+//   HF: p_i += sum_d exp(i kappa d.(x_i-c)) sum_I S_I((x_i-c)/r) sig2_d[I] L_d[I]
+//   LF: p_i += sum_I S_I((x_i-c)/r) L[I]
+template <typename R, int L>
+__global__ __launch_bounds__(64) void k_l2p(const int* __restrict__ leaves, DevCells T,
+                                            DevLevels lv, const double* __restrict__ px,
+                                            const double* __restrict__ py,
+                                            const double* __restrict__ pz,
+                                            const double* __restrict__ dirs,
+                                            const typename C2T<R>::T* __restrict__ sig2,
+                                            const typename C2T<R>::T* __restrict__ Lst,
+                                            OpConsts<R> oc, double kappa,
+                                            typename C2T<R>::T* __restrict__ p) {
+  typedef typename C2T<R>::T R2;
+  constexpr int LL = L * L * L;
+  __shared__ R2 buf[LL];
+  const int c = leaves[blockIdx.x];
+  const int lvl = T.lvl[c], pb = T.pbeg[c], n = T.nprt[c], s0 = T.slot0[c], ns = T.nslot[c];
+  const double cx = T.ctr[3 * c], cy = T.ctr[3 * c + 1], cz = T.ctr[3 * c + 2], rad = T.rad[c];
+  const int hf = lv.isHF[lvl];
+  const int doff = lv.dirOff[lvl], srow = lv.sigRow0[lvl];
+  const int tid = threadIdx.x;
+  for (int i0 = 0; i0 < n; i0 += 64) {
+    const int i = i0 + tid;
+    const bool act = i < n;
+    double r0 = 0., r1 = 0., r2 = 0.;
+    if (act) {
+      r0 = px[pb + i] - cx;
+      r1 = py[pb + i] - cy;
+      r2 = pz[pb + i] - cz;
+    }
+    R sx[LMAX], sy[LMAX], sz[LMAX];
+    lagrangeAll<R>((R)(r0 / rad), L, oc.xi, oc.iden, sx);
+    lagrangeAll<R>((R)(r1 / rad), L, oc.xi, oc.iden, sy);
+    lagrangeAll<R>((R)(r2 / rad), L, oc.xi, oc.iden, sz);
+    R2 pacc = c2<R>(0, 0);
+    for (int k = 0; k < ns; k++) {
+      const int d = hf ? T.slotDir[s0 + k] : 0;
+      __syncthreads();
+      for (int I = tid; I < LL; I += 64) {
+        R2 v = Lst[(size_t)(s0 + k) * LL + I];
+        if (hf) v = cmul(v, sig2[(size_t)(srow + d) * LL + I]);
+        buf[I] = v;
+      }
+      __syncthreads();
+      R2 s = c2<R>(0, 0);
+#pragma unroll
+      for (int a = 0; a < L; a++)
+#pragma unroll
+        for (int b = 0; b < L; b++) {
+          R2 s1 = c2<R>(0, 0);
+#pragma unroll
+          for (int e = 0; e < L; e++) {
+            const R2 v = buf[(a * L + b) * L + e];
+            s1.x += sz[e] * v.x;
+            s1.y += sz[e] * v.y;
+          }
+          const R t = sx[a] * sy[b];
+          s.x += t * s1.x;
+          s.y += t * s1.y;
+        }
+      if (hf) {
+        const double dx = dirs[3 * (doff + d)], dy = dirs[3 * (doff + d) + 1],
+                     dz = dirs[3 * (doff + d) + 2];
+        R cs, sn;
+        expi(kappa * (dx * r0 + dy * r1 + dz * r2), cs, sn);
+        s = cmul(s, c2<R>(cs, sn));
+      }
+      pacc.x += s.x;
+      pacc.y += s.y;
+    }
+    if (act) {
+      R2 o = p[pb + i];
+      o.x += pacc.x;
+      o.y += pacc.y;
+      p[pb + i] = o;
+    }
+  }
+}
+
+// P2P near field (P2P.hpp:42-79): p_i += sum_j exp(i kappa r)/(4 pi r) q_j,
+// r = |x_i - y_j|, term 0 when r^2 < 1e-16.  A work item is up to 64 targets
+// of one leaf against a run of its source list; with nt targets, the wave's
+// lanes form floor(64/nt) groups that split the sources.  Sources are staged
+// in LDS 64 at a time.  Each item writes per-target partial sums (double).
+struct P2PItem {
+  int leaf;  // index into the leaf list
+  int t0;    // first target of the chunk (leaf-relative)
+  int r;     // first source range
+  int off;   // offset into that range
+  int cnt;   // number of sources
+};
+
+template <typename R>
+__device__ __forceinline__ void p2pTerm(double xt, double yt, double zt, double xs, double ys,
+                                        double zs, typename C2T<R>::T qs, double kappa,
+                                        R& pr, R& pi);
+template <>
+__device__ __forceinline__ void p2pTerm<double>(double xt, double yt, double zt, double xs,
+                                                double ys, double zs, double2 qs, double kappa,
+                                                double& pr, double& pi) {
+  const double x = xt - xs, y = yt - ys, z = zt - zs;
+  const double R2v = x * x + y * y + z * z;
+  double K = rsqrt(R2v);
+  if (R2v < 1.e-16) K = 0.;
+  const double r = K * R2v * kappa;
+  K *= 0.079577471545947667884;  // 1/(4 pi)
+  double si, co;
+  sincos(r, &si, &co);
+  co *= K;
+  si *= K;
+  pr += co * qs.x - si * qs.y;
+  pi += co * qs.y + si * qs.x;
+}
+template <>
+__device__ __forceinline__ void p2pTerm<float>(double xt, double yt, double zt, double xs,
+                                               double ys, double zs, float2 qs, double kappa,
+                                               float& pr, float& pi) {
+  const double x = xt - xs, y = yt - ys, z = zt - zs;
+  const double R2v = x * x + y * y + z * z;
+  // r to ~1e-14 relative: float reciprocal square root, one Newton step in double
+  const float y0 = (R2v < 1.e-16) ? 0.f : rsqrtf((float)R2v);
+  const double r0 = R2v * (double)y0;
+  const double r = fma(0.5 * (double)y0, fma(-r0, r0, R2v), r0);
+  const float K = y0 * 0.079577471545947667884f;
+  float co, si;
+  expi(kappa * r, co, si);
+  co *= K;
+  si *= K;
+  pr += co * qs.x - si * qs.y;
+  pi += co * qs.y + si * qs.x;
+}
+
+template <typename R>
+__global__ __launch_bounds__(64) void k_p2p(const P2PItem* __restrict__ items,
+                                            const int* __restrict__ leafCell,
+                                            const int* __restrict__ tpbeg,
+                                            const int* __restrict__ tnprt,
+                                            const double* __restrict__ tx,
+                                            const double* __restrict__ ty,
+                                            const double* __restrict__ tz,
+                                            const double* __restrict__ sx,
+                                            const double* __restrict__ sy,
+                                            const double* __restrict__ sz,
+                                            const typename C2T<R>::T* __restrict__ q,
+                                            const int* __restrict__ rb, const int* __restrict__ re,
+                                            double kappa, double2* __restrict__ partial) {
+  typedef typename C2T<R>::T R2;
+  __shared__ double lx[64], ly[64], lz[64];
+  __shared__ R2 lq[64];
+  __shared__ double2 red[64];
+  const P2PItem it = items[blockIdx.x];
+  const int c = leafCell[it.leaf];
+  const int tb = tpbeg[c] + it.t0;
+  const int nt = min(64, tnprt[c] - it.t0);
+  const int G = 64 / nt;
+  const int lane = threadIdx.x;
+  const int ti = lane % nt, g = lane / nt;
+  const bool active = g < G;
+  const double xt = tx[tb + ti], yt = ty[tb + ti], zt = tz[tb + ti];
+  double accr = 0., acci = 0.;
+  int r = it.r, off = it.off;
+  for (int pos = 0; pos < it.cnt; pos += 64) {
+    const int tileN = min(64, it.cnt - pos);
+    __syncthreads();
+    if (lane < tileN) {
+      int rr = r, oo = off + lane;
+      int len = re[rr] - rb[rr];
+      while (oo >= len) {
+        oo -= len;
+        rr++;
+        len = re[rr] - rb[rr];
+      }
+      const int j = rb[rr] + oo;
+      lx[lane] = sx[j];
+      ly[lane] = sy[j];
+      lz[lane] = sz[j];
+      lq[lane] = q[j];
+    }
+    {
+      int adv = tileN;
+      while (adv > 0) {
+        const int rem = re[r] - rb[r] - off;
+        if (adv < rem) {
+          off += adv;
+          adv = 0;
+        } else {
+          adv -= rem;
+          r++;
+          off = 0;
+        }
+      }
+    }
+    __syncthreads();
+    if (active) {
+      R pr = 0, pi = 0;
+      for (int j = g; j < tileN; j += G) p2pTerm<R>(xt, yt, zt, lx[j], ly[j], lz[j], lq[j], kappa, pr, pi);
+      accr += (double)pr;
+      acci += (double)pi;
+    }
+  }
+  red[lane] = make_double2(accr, acci);
+  __syncthreads();
+  if (lane < nt) {
+    double2 s = red[lane];
+    for (int gg = 1; gg < G; gg++) {
+      s.x += red[gg * nt + lane].x;
+      s.y += red[gg * nt + lane].y;
+    }
+    partial[(size_t)blockIdx.x * 64 + lane] = s;
+  }
+}
+
+// Sum the partial results of a leaf's items into p (items of a leaf are
+// contiguous, ordered by target chunk).
+template <typename R>
+__global__ __launch_bounds__(64) void k_p2p_reduce(const int* __restrict__ itemBeg,
+                                                   const P2PItem* __restrict__ items,
+                                                   const int* __restrict__ leafCell,
+                                                   const int* __restrict__ tpbeg,
+                                                   const int* __restrict__ tnprt,
+                                                   const double2* __restrict__ partial,
+                                                   typename C2T<R>::T* __restrict__ p) {
+  typedef typename C2T<R>::T R2;
+  const int leaf = blockIdx.x;
+  const int c = leafCell[leaf];
+  const int pb = tpbeg[c], n = tnprt[c];
+  const int ib = itemBeg[leaf], ie = itemBeg[leaf + 1];
+  for (int t0 = 0; t0 < n; t0 += 64) {
+    const int i = t0 + threadIdx.x;
+    double sr = 0., si = 0.;
+    for (int k = ib; k < ie; k++) {
+      if (items[k].t0 != t0) continue;
+      const double2 v = partial[(size_t)k * 64 + threadIdx.x];
+      sr += v.x;
+      si += v.y;
+    }
+    if (i < n) {
+      R2 o = p[pb + i];
+      o.x = (R)((double)o.x + sr);
+      o.y = (R)((double)o.y + si);
+      p[pb + i] = o;
+    }
+  }
+}
+
+// FP64 direct sum at a list of targets (the check of include/interface.hpp:
+// 178-221 on sampled targets): one thread per target, sources staged in LDS,
+// compensated (Kahan) accumulation.
+__global__ __launch_bounds__(256) void k_direct(const double* __restrict__ src, int ns,
+                                                const double2* __restrict__ q,
+                                                const double* __restrict__ trg, int nt,
+                                                double kappa, double2* __restrict__ out) {
+  __shared__ double sx[256], sy[256], sz[256];
+  __shared__ double2 sq[256];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const bool act = i < nt;
+  double xt = 0., yt = 0., zt = 0.;
+  if (act) {
+    xt = trg[3 * i];
+    yt = trg[3 * i + 1];
+    zt = trg[3 * i + 2];
+  }
+  double pr = 0., pi = 0., cr = 0., ci = 0.;
+  for (int j0 = 0; j0 < ns; j0 += 256) {
+    const int n = min(256, ns - j0);
+    __syncthreads();
+    if (threadIdx.x < n) {
+      const int j = j0 + threadIdx.x;
+      sx[threadIdx.x] = src[3 * j];
+      sy[threadIdx.x] = src[3 * j + 1];
+      sz[threadIdx.x] = src[3 * j + 2];
+      sq[threadIdx.x] = q[j];
+    }
+    __syncthreads();
+    if (act)
+      for (int j = 0; j < n; j++) {
+        const double x = xt - sx[j], y = yt - sy[j], z = zt - sz[j];
+        const double R = x * x + y * y + z * z;
+        if (R < 1.e-16) continue;
+        const double r = sqrt(R);
+        const double K = 0.079577471545947667884 / r;
+        double si, co;
+        sincos(kappa * r, &si, &co);
+        co *= K;
+        si *= K;
+        const double tr = co * sq[j].x - si * sq[j].y - cr;
+        const double s1 = pr + tr;
+        cr = (s1 - pr) - tr;
+        pr = s1;
+        const double ti = co * sq[j].y + si * sq[j].x - ci;
+        const double s2 = pi + ti;
+        ci = (s2 - pi) - ti;
+        pi = s2;
+      }
+  }
+  if (act) out[i] = make_double2(pr, pi);
+}
+
+// ----------------------------------------------------------------------------
+// executor
+// ----------------------------------------------------------------------------
+
+#define HIPCK(x)                                                              \
+  do {                                                                        \
+    hipError_t e_ = (x);                                                      \
+    if (e_ != hipSuccess) {                                                   \
+      g_err = std::string(#x) + ": " + hipGetErrorString(e_);                 \
+      return e_ == hipErrorOutOfMemory ? ERR_MEMORY : ERR_HIP;                \
+    }                                                                         \
+  } while (0)
+
+struct DevTrans {
+  int n = 0;  // output slots
+  int level = 0, hf = 0;
+  int *out = nullptr, *beg = nullptr, *in = nullptr, *sgn = nullptr, *sigIdx = nullptr;
+  double2* edv = nullptr;
+};
+
+struct Exec {
+  int prec = 0;
+  int L = 0, P = 0, LL = 0, PP = 0;
+  double kappa = 0;
+  int64_t ns = 0, nt = 0;
+  Stats st;
+  int device = 0;
+  std::vector<void*> mem;
+  int64_t bytes = 0;
+  hipStream_t own = nullptr;
+  hipEvent_t ev[DEFMM_NSTAGES + 1] = {};
+  bool evRecorded = false;
+
+  double *sx = nullptr, *sy = nullptr, *sz = nullptr, *tx = nullptr, *ty = nullptr, *tz = nullptr;
+  int *sperm = nullptr, *tperm = nullptr;
+  DevCells S{}, T{};
+  DevLevels lv{};
+  double* dirs = nullptr;
+  void *sig = nullptr, *sig2 = nullptr, *U = nullptr;
+  int* perm = nullptr;
+  void* consts = nullptr;  // xi, iden, A0, A1, B0, B1, twc, tws (real type)
+  int *p2m = nullptr, *l2p = nullptr;
+  int np2m = 0, nl2p = 0;
+  std::vector<DevTrans> m2m, l2l;
+  int *m2fSlot = nullptr;
+  int nm2f = 0;
+  int *m2lBeg = nullptr, *m2lSrcF = nullptr, *m2lTabHash = nullptr;
+  int nTslots = 0, nSslots = 0;
+  P2PItem* items = nullptr;
+  int nitems = 0;
+  int *leafCell = nullptr, *itemBeg = nullptr, *rb = nullptr, *re = nullptr;
+  int nleaves = 0;
+  double2* partial = nullptr;
+  void *q = nullptr, *p = nullptr, *ML = nullptr, *F = nullptr;
+  double2 *qio = nullptr, *pio = nullptr;
+  double stageMs[DEFMM_NSTAGES] = {};
+
+  ~Exec() {
+    for (void* m : mem) hipFree(m);
+    for (auto& e : ev)
+      if (e) hipEventDestroy(e);
+    if (own) hipStreamDestroy(own);
+  }
+
+  template <class X>
+  int up(X*& dst, const X* src, size_t n) {
+    dst = nullptr;
+    if (n == 0) n = 1;
+    void* d = nullptr;
+    HIPCK(hipMalloc(&d, n * sizeof(X)));
+    mem.push_back(d);
+    bytes += (int64_t)(n * sizeof(X));
+    dst = (X*)d;
+    if (src) HIPCK(hipMemcpy(d, src, n * sizeof(X), hipMemcpyHostToDevice));
+    return OK;
+  }
+  template <class X>
+  int upv(X*& dst, const std::vector<X>& v) {
+    return up(dst, v.empty() ? nullptr : v.data(), v.size());
+  }
+  int alloc(void*& dst, size_t nbytes) {
+    void* d = nullptr;
+    if (nbytes == 0) nbytes = 16;
+    HIPCK(hipMalloc(&d, nbytes));
+    mem.push_back(d);
+    bytes += (int64_t)nbytes;
+    dst = d;
+    return OK;
+  }
+};
+
+template <typename R>
+static std::vector<typename C2T<R>::T> toR2(const std::vector<cplx>& v) {
+  std::vector<typename C2T<R>::T> o(v.size());
+  for (size_t i = 0; i < v.size(); i++) {
+    o[i].x = (R)v[i].real();
+    o[i].y = (R)v[i].imag();
+  }
+  return o;
+}
+
+#define CKR(x)               \
+  do {                       \
+    int r_ = (x);            \
+    if (r_ != OK) return r_; \
+  } while (0)
+
+template <typename R>
+static int uploadTyped(Exec& X, Plan& pl) {
+  typedef typename C2T<R>::T R2;
+  const int L = pl.L, P = pl.P;
+  {
+    auto a = toR2<R>(pl.sig);
+    R2* d;
+    CKR(X.upv(d, a));
+    X.sig = d;
+    a = toR2<R>(pl.sig2);
+    CKR(X.upv(d, a));
+    X.sig2 = d;
+    a = toR2<R>(pl.m2lTab);
+    CKR(X.upv(d, a));
+    X.U = d;
+  }
+  {
+    std::vector<R> c;
+    std::vector<double> xi(L), iden(L);
+    for (int i = 0; i < L; i++) xi[i] = -1. + 2. * ((double)i / (double)(L - 1));
+    for (int i = 0; i < L; i++) {
+      double pr = 1.;
+      for (int j = 0; j < L; j++)
+        if (j != i) pr *= (xi[i] - xi[j]);
+      iden[i] = 1. / pr;
+    }
+    for (double v : xi) c.push_back((R)v);
+    for (double v : iden) c.push_back((R)v);
+    for (int s = 0; s < 2; s++)
+      for (double v : pl.A[s]) c.push_back((R)v);
+    for (int s = 0; s < 2; s++)
+      for (double v : pl.B[s]) c.push_back((R)v);
+    for (int m = 0; m < P; m++) c.push_back((R)std::cos(2. * M_PI * (double)m / (double)P));
+    for (int m = 0; m < P; m++) c.push_back((R)std::sin(2. * M_PI * (double)m / (double)P));
+    R* d;
+    CKR(X.upv(d, c));
+    X.consts = d;
+  }
+  size_t slots = (size_t)std::max(pl.S.nslots, pl.T.nslots);
+  CKR(X.alloc(X.ML, slots * pl.LL * sizeof(R2)));
+  CKR(X.alloc(X.F, (size_t)pl.m2fSlot.size() * pl.PP * sizeof(R2)));
+  CKR(X.alloc(X.q, (size_t)X.ns * sizeof(R2)));
+  CKR(X.alloc(X.p, (size_t)X.nt * sizeof(R2)));
+  return OK;
+}
+
+static int upload(Exec& X, Plan& pl) {
+  X.L = pl.L; X.P = pl.P; X.LL = pl.LL; X.PP = pl.PP; X.kappa = pl.kappa;
+  X.st = pl.st;
+  X.ns = (int64_t)pl.S.x.size();
+  X.nt = (int64_t)pl.T.x.size();
+  CKR(X.upv(X.sx, pl.S.x)); CKR(X.upv(X.sy, pl.S.y)); CKR(X.upv(X.sz, pl.S.z));
+  CKR(X.upv(X.tx, pl.T.x)); CKR(X.upv(X.ty, pl.T.y)); CKR(X.upv(X.tz, pl.T.z));
+  CKR(X.upv(X.sperm, pl.S.perm)); CKR(X.upv(X.tperm, pl.T.perm));
+  for (int w = 0; w < 2; w++) {
+    Tree& t = w ? pl.T : pl.S;
+    DevCells& d = w ? X.T : X.S;
+    int* ip;
+    double* dp;
+    CKR(X.upv(ip, t.lvl)); d.lvl = ip;
+    CKR(X.upv(ip, t.pbeg)); d.pbeg = ip;
+    CKR(X.upv(ip, t.nprt)); d.nprt = ip;
+    CKR(X.upv(dp, t.ctr)); d.ctr = dp;
+    CKR(X.upv(dp, t.rad)); d.rad = dp;
+    CKR(X.upv(ip, t.slot0)); d.slot0 = ip;
+    CKR(X.upv(ip, t.nslot)); d.nslot = ip;
+    CKR(X.upv(ip, t.slotDir)); d.slotDir = ip;
+  }
+  X.nSslots = pl.S.nslots;
+  X.nTslots = pl.T.nslots;
+  {
+    std::vector<int> dirOff(pl.nlvl, 0);
+    std::vector<double> dd;
+    std::vector<int> stageOff;
+    for (size_t e = 0; e < pl.dirs.size(); e++) {
+      stageOff.push_back((int)(dd.size() / 3));
+      for (auto& v : pl.dirs[e]) { dd.push_back(v[0]); dd.push_back(v[1]); dd.push_back(v[2]); }
+    }
+    for (int l = 0; l < pl.nlvl; l++)
+      if (pl.isHF[l] && pl.HFStage[l] < (int)stageOff.size()) dirOff[l] = stageOff[pl.HFStage[l]];
+    int* ip;
+    CKR(X.upv(X.dirs, dd));
+    CKR(X.upv(ip, pl.isHF)); X.lv.isHF = ip;
+    CKR(X.upv(ip, dirOff)); X.lv.dirOff = ip;
+    CKR(X.upv(ip, pl.sigRow0)); X.lv.sigRow0 = ip;
+  }
+  CKR(X.upv(X.perm, pl.perm));
+  CKR(X.upv(X.p2m, pl.p2mCell)); X.np2m = (int)pl.p2mCell.size();
+  CKR(X.upv(X.l2p, pl.l2pCell)); X.nl2p = (int)pl.l2pCell.size();
+  for (int w = 0; w < 2; w++) {
+    auto& src = w ? pl.l2l : pl.m2m;
+    auto& dst = w ? X.l2l : X.m2m;
+    for (auto& tl : src) {
+      DevTrans d;
+      d.n = (int)tl.out.size();
+      d.level = tl.level;
+      d.hf = tl.hf;
+      CKR(X.upv(d.out, tl.out)); CKR(X.upv(d.beg, tl.beg)); CKR(X.upv(d.in, tl.in));
+      CKR(X.upv(d.sgn, tl.sgn)); CKR(X.upv(d.sigIdx, tl.sigIdx));
+      std::vector<double2> e(tl.edv.size());
+      for (size_t k = 0; k < e.size(); k++) e[k] = make_double2(tl.edv[k].real(), tl.edv[k].imag());
+      CKR(X.upv(d.edv, e));
+      dst.push_back(d);
+    }
+  }
+  CKR(X.upv(X.m2fSlot, pl.m2fSlot)); X.nm2f = (int)pl.m2fSlot.size();
+  CKR(X.upv(X.m2lBeg, pl.m2lBeg)); CKR(X.upv(X.m2lSrcF, pl.m2lSrcF));
+  {
+    std::vector<int> th(pl.m2lTabId.size());
+    for (size_t k = 0; k < th.size(); k++) th[k] = (pl.m2lTabId[k] << 6) | pl.m2lHash[k];
+    CKR(X.upv(X.m2lTabHash, th));
+  }
+  // P2P work items: 64-target chunks x runs of <= kBudget sources
+  {
+    const int kBudget = 4096;
+    std::vector<P2PItem> items;
+    std::vector<int> itemBeg;
+    const Tree& T = pl.T;
+    for (size_t lf = 0; lf < pl.p2pLeaf.size(); lf++) {
+      itemBeg.push_back((int)items.size());
+      int c = pl.p2pLeaf[lf];
+      for (int t0 = 0; t0 < T.nprt[c]; t0 += 64) {
+        int r = pl.p2pBeg[lf], off = 0, cnt = 0, r0 = r, off0 = 0;
+        for (; r < pl.p2pBeg[lf + 1]; r++) {
+          int len = pl.p2pSrcE[r] - pl.p2pSrcB[r];
+          off = 0;
+          while (off < len) {
+            int take = std::min(len - off, kBudget - cnt);
+            if (cnt == 0) { r0 = r; off0 = off; }
+            cnt += take;
+            off += take;
+            if (cnt == kBudget) {
+              items.push_back({(int)lf, t0, r0, off0, cnt});
+              cnt = 0;
+            }
+          }
+        }
+        if (cnt) items.push_back({(int)lf, t0, r0, off0, cnt});
+      }
+    }
+    itemBeg.push_back((int)items.size());
+    // the kernel walks ranges from (r, off) for cnt sources: they must stay
+    // inside the item's leaf list
+    for (const P2PItem& it : items) {
+      int rEnd = pl.p2pBeg[it.leaf + 1];
+      if (it.r < pl.p2pBeg[it.leaf] || it.r >= rEnd || it.cnt < 1 || it.t0 < 0 ||
+          it.t0 >= T.nprt[pl.p2pLeaf[it.leaf]]) {
+        g_err = "internal: P2P item out of range";
+        return ERR_STATE;
+      }
+      int64_t avail = -(int64_t)it.off;
+      for (int r = it.r; r < rEnd; r++) avail += pl.p2pSrcE[r] - pl.p2pSrcB[r];
+      if (it.off < 0 || it.off >= pl.p2pSrcE[it.r] - pl.p2pSrcB[it.r] || it.cnt > avail) {
+        g_err = "internal: P2P item exceeds its source list";
+        return ERR_STATE;
+      }
+    }
+    X.nitems = (int)items.size();
+    X.nleaves = (int)pl.p2pLeaf.size();
+    X.st.p2p_items = X.nitems;
+    CKR(X.upv(X.items, items));
+    CKR(X.upv(X.itemBeg, itemBeg));
+    CKR(X.upv(X.leafCell, pl.p2pLeaf));
+    CKR(X.upv(X.rb, pl.p2pSrcB));
+    CKR(X.upv(X.re, pl.p2pSrcE));
+    void* pp;
+    CKR(X.alloc(pp, (size_t)std::max(1, X.nitems) * 64 * sizeof(double2)));
+    X.partial = (double2*)pp;
+  }
+  {
+    void* d;
+    CKR(X.alloc(d, (size_t)X.ns * sizeof(double2)));
+    X.qio = (double2*)d;
+    CKR(X.alloc(d, (size_t)X.nt * sizeof(double2)));
+    X.pio = (double2*)d;
+  }
+  if (X.prec == DEFMM_FP64) return uploadTyped<double>(X, pl);
+  return uploadTyped<float>(X, pl);
+}
+
+static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+template <typename R, int L>
+static int runTyped(Exec& X, const double2* qin, double2* pout, int parts, hipStream_t s) {
+  typedef typename C2T<R>::T R2;
+  constexpr int P = 2 * L - 1;
+  const R* cst = (const R*)X.consts;
+  OpConsts<R> oc;
+  oc.xi = cst;
+  oc.iden = cst + L;
+  oc.A[0] = cst + 2 * L;
+  oc.A[1] = oc.A[0] + L * L;
+  oc.B[0] = oc.A[1] + L * L;
+  oc.B[1] = oc.B[0] + L * L;
+  oc.twc = oc.B[1] + L * L;
+  oc.tws = oc.twc + P;
+  R2* q = (R2*)X.q;
+  R2* p = (R2*)X.p;
+  R2* ML = (R2*)X.ML;
+  R2* F = (R2*)X.F;
+  const R2* sig = (const R2*)X.sig;
+  const R2* sig2 = (const R2*)X.sig2;
+  auto mark = [&](int k) { hipEventRecord(X.ev[k], s); };
+  mark(0);
+  hipLaunchKernelGGL(k_zero<R2>, dim3(nblk(X.nt, 256)), dim3(256), 0, s, p, (int64_t)X.nt);
+  hipLaunchKernelGGL(k_gather<R>, dim3(nblk(X.ns, 256)), dim3(256), 0, s, qin, X.sperm, q, (int)X.ns);
+  mark(1);
+  const bool far = (parts & 2) != 0, near = (parts & 1) != 0;
+  if (far && X.np2m)
+    hipLaunchKernelGGL((k_p2m<R, L>), dim3(X.np2m), dim3(128), 0, s, X.p2m, X.S, X.lv, X.sx, X.sy,
+                       X.sz, (const R2*)q, X.dirs, sig2, oc, X.kappa, ML);
+  mark(2);
+  if (far)
+    for (auto& t : X.m2m)
+      hipLaunchKernelGGL((k_trans<R, L>), dim3(t.n), dim3(128), 0, s, t.out, t.beg, t.in, t.sgn,
+                         t.sigIdx, t.edv, (const R2*)ML, ML, sig, sig2, oc.A[0], oc.A[1], 0);
+  mark(3);
+  if (far && X.nm2f)
+    hipLaunchKernelGGL((k_m2f<R, L>), dim3(X.nm2f), dim3(256), 0, s, X.m2fSlot, (const R2*)ML, F,
+                       oc.twc, oc.tws);
+  mark(4);
+  // L (aliasing M) is written for every target slot here
+  if (far && X.nTslots)
+    hipLaunchKernelGGL((k_m2l<R, L>), dim3(X.nTslots), dim3(256), 0, s, X.m2lBeg, X.m2lSrcF,
+                       X.m2lTabHash, (const R2*)F, (const R2*)X.U, X.perm, ML, oc.twc, oc.tws);
+  mark(5);
+  if (far)
+    for (auto& t : X.l2l)
+      hipLaunchKernelGGL((k_trans<R, L>), dim3(t.n), dim3(128), 0, s, t.out, t.beg, t.in, t.sgn,
+                         t.sigIdx, t.edv, (const R2*)ML, ML, sig2, sig, oc.B[0], oc.B[1], 1);
+  mark(6);
+  if (far && X.nl2p)
+    hipLaunchKernelGGL((k_l2p<R, L>), dim3(X.nl2p), dim3(64), 0, s, X.l2p, X.T, X.lv, X.tx, X.ty,
+                       X.tz, X.dirs, sig2, (const R2*)ML, oc, X.kappa, p);
+  mark(7);
+  if (near && X.nitems) {
+    hipLaunchKernelGGL(k_p2p<R>, dim3(X.nitems), dim3(64), 0, s, X.items, X.leafCell, X.T.pbeg,
+                       X.T.nprt, X.tx, X.ty, X.tz, X.sx, X.sy, X.sz, (const R2*)q, X.rb, X.re,
+                       X.kappa, X.partial);
+    hipLaunchKernelGGL(k_p2p_reduce<R>, dim3(X.nleaves), dim3(64), 0, s, X.itemBeg, X.items,
+                       X.leafCell, X.T.pbeg, X.T.nprt, (const double2*)X.partial, p);
+  }
+  mark(8);
+  hipLaunchKernelGGL(k_scatter<R>, dim3(nblk(X.nt, 256)), dim3(256), 0, s, (const R2*)p, X.tperm,
+                     pout, (int)X.nt);
+  mark(9);
+  X.evRecorded = true;
+  HIPCK(hipGetLastError());
+  return OK;
+}
+
+template <typename R>
+static int runR(Exec& X, const double2* qin, double2* pout, int parts, hipStream_t s) {
+  switch (X.L) {
+    case 2: return runTyped<R, 2>(X, qin, pout, parts, s);
+    case 3: return runTyped<R, 3>(X, qin, pout, parts, s);
+    case 4: return runTyped<R, 4>(X, qin, pout, parts, s);
+    case 5: return runTyped<R, 5>(X, qin, pout, parts, s);
+    case 6: return runTyped<R, 6>(X, qin, pout, parts, s);
+    case 7: return runTyped<R, 7>(X, qin, pout, parts, s);
+    case 8: return runTyped<R, 8>(X, qin, pout, parts, s);
+  }
+  g_err = "interpolation order outside 2..8";
+  return ERR_ARG;
+}
+
+static int run(Exec& X, const double2* qin, double2* pout, int parts, hipStream_t s) {
+  HIPCK(hipSetDevice(X.device));
+  return X.prec == DEFMM_FP64 ? runR<double>(X, qin, pout, parts, s)
+                              : runR<float>(X, qin, pout, parts, s);
+}
+
+}  // namespace dfm
+
+// ----------------------------------------------------------------------------
+// C ABI
+// ----------------------------------------------------------------------------
+using dfm::Exec;
+using dfm::g_err;
+
+struct defmm_plan_s {
+  Exec x;
+};
+
+extern "C" {
+
+const char* defmm_strerror(int code) {
+  switch (code) {
+    case DEFMM_OK: return "success";
+    case DEFMM_EARG: return "invalid argument";
+    case DEFMM_ENOPRECOMP: return "no M2L level: tree too shallow for this wavenumber";
+    case DEFMM_EDEPTH: return "octree deeper than 30 levels";
+    case DEFMM_EHIP: return "HIP runtime error";
+    case DEFMM_EMEMORY: return "out of memory";
+    case DEFMM_ETABLE: return "M2L offset outside the precomputed tables";
+    case DEFMM_ENODEVICE: return "no GPU visible";
+    case DEFMM_ESTATE: return "call out of sequence";
+  }
+  return "unknown error";
+}
+
+const char* defmm_last_error(void) { return g_err.c_str(); }
+
+int defmm_plan_create(const double* src_xyz, int64_t n_src, const double* trg_xyz,
+                      int64_t n_trg, int order, int ncrit, double kappa, int lvmax,
+                      int precision, defmm_plan* out) {
+  if (!out) { g_err = "null output pointer"; return DEFMM_EARG; }
+  *out = nullptr;
+  if (precision != DEFMM_FP64 && precision != DEFMM_MIXED) { g_err = "bad precision"; return DEFMM_EARG; }
+  if (order < 2 || order > dfm::LMAX) { g_err = "order must be in 2..8"; return DEFMM_EARG; }
+  if (lvmax < -1 || (lvmax < 0 && ncrit < 1)) { g_err = "bad ncrit/lvmax"; return DEFMM_EARG; }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) { g_err = "no HIP device"; return DEFMM_ENODEVICE; }
+  dfm::Plan* pl = new dfm::Plan();
+  std::string msg;
+  int rc = dfm::buildPlan(*pl, src_xyz, n_src, trg_xyz, n_trg, order, lvmax >= 0 ? -lvmax : ncrit,
+                          kappa, msg);
+  if (rc != dfm::OK) {
+    g_err = msg;
+    delete pl;
+    return rc;
+  }
+  defmm_plan_s* h = new defmm_plan_s();
+  h->x.prec = precision;
+  hipGetDevice(&h->x.device);
+  rc = dfm::upload(h->x, *pl);
+  delete pl;
+  if (rc == dfm::OK) {
+    if (hipStreamCreate(&h->x.own) != hipSuccess) rc = DEFMM_EHIP;
+    for (auto& e : h->x.ev)
+      if (rc == dfm::OK && hipEventCreate(&e) != hipSuccess) rc = DEFMM_EHIP;
+  }
+  if (rc != dfm::OK) {
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return DEFMM_OK;
+}
+
+int defmm_execute_device(defmm_plan plan, const void* q_dev, void* p_dev, void* stream) {
+  if (!plan || !q_dev || !p_dev) { g_err = "null argument"; return DEFMM_EARG; }
+  return dfm::run(plan->x, (const double2*)q_dev, (double2*)p_dev, 3, (hipStream_t)stream);
+}
+
+static int execHost(defmm_plan plan, const double* q, double* p, int parts) {
+  if (!plan || !q || !p) { g_err = "null argument"; return DEFMM_EARG; }
+  if (parts < 1 || parts > 3) { g_err = "parts must be 1, 2 or 3"; return DEFMM_EARG; }
+  Exec& X = plan->x;
+  if (hipSetDevice(X.device) != hipSuccess) { g_err = "hipSetDevice failed"; return DEFMM_EHIP; }
+  if (hipMemcpyAsync(X.qio, q, X.ns * sizeof(double2), hipMemcpyHostToDevice, X.own) != hipSuccess) {
+    g_err = "copy of charges failed";
+    return DEFMM_EHIP;
+  }
+  int rc = dfm::run(X, X.qio, X.pio, parts, X.own);
+  if (rc) return rc;
+  if (hipMemcpyAsync(p, X.pio, X.nt * sizeof(double2), hipMemcpyDeviceToHost, X.own) != hipSuccess ||
+      hipStreamSynchronize(X.own) != hipSuccess) {
+    g_err = std::string("execution failed: ") + hipGetErrorString(hipGetLastError());
+    return DEFMM_EHIP;
+  }
+  return DEFMM_OK;
+}
+
+int defmm_execute(defmm_plan plan, const double* q, double* p) { return execHost(plan, q, p, 3); }
+
+int defmm_execute_parts(defmm_plan plan, const double* q, double* p, int parts) {
+  return execHost(plan, q, p, parts);
+}
+
+int defmm_plan_stats(defmm_plan plan, int64_t* out, int n) {
+  if (!plan || !out) { g_err = "null argument"; return DEFMM_EARG; }
+  const dfm::Stats& s = plan->x.st;
+  int64_t v[DEFMM_NSTATS] = {s.p2p_calls, s.p2p_pairs, s.hf_m2l, s.lf_m2l, s.src_slots,
+                             s.trg_slots, s.hf_tables, s.lf_tables, s.hf_leaves_p2m,
+                             s.lf_leaves_p2m, s.hf_m2m, s.lf2hf_m2m, s.lf_m2m, s.hf_l2p,
+                             s.lf_l2p, s.hf_l2l, s.hf2lf_l2l, s.lf_l2l, s.nlvl, s.cpt,
+                             s.max_stage, s.ncells_t, s.ncells_s, s.p2p_items, s.m2f_slots,
+                             s.m2l_entries, s.m2l_targets};
+  for (int i = 0; i < n && i < DEFMM_NSTATS; i++) out[i] = v[i];
+  return DEFMM_OK;
+}
+
+int defmm_stage_times(defmm_plan plan, double* ms, int n) {
+  if (!plan || !ms) { g_err = "null argument"; return DEFMM_EARG; }
+  Exec& X = plan->x;
+  if (!X.evRecorded) { g_err = "no execution recorded"; return DEFMM_ESTATE; }
+  if (hipEventSynchronize(X.ev[DEFMM_NSTAGES]) != hipSuccess) { g_err = "event sync failed"; return DEFMM_EHIP; }
+  for (int k = 0; k < DEFMM_NSTAGES && k < n; k++) {
+    float t = 0.f;
+    if (hipEventElapsedTime(&t, X.ev[k], X.ev[k + 1]) != hipSuccess) { g_err = "event timing failed"; return DEFMM_EHIP; }
+    ms[k] = t;
+  }
+  return DEFMM_OK;
+}
+
+int defmm_plan_device_bytes(defmm_plan plan, int64_t* bytes) {
+  if (!plan || !bytes) { g_err = "null argument"; return DEFMM_EARG; }
+  *bytes = plan->x.bytes;
+  return DEFMM_OK;
+}
+
+int defmm_direct(const double* src_xyz, int64_t n_src, const double* q, const double* trg_xyz,
+                 int64_t n_trg, double kappa, double* p) {
+  if (!src_xyz || !q || !trg_xyz || !p || n_src < 1 || n_trg < 1 || n_src > 2000000000 ||
+      n_trg > 2000000000) {
+    g_err = "invalid argument";
+    return DEFMM_EARG;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) { g_err = "no HIP device"; return DEFMM_ENODEVICE; }
+  void *ds = nullptr, *dq = nullptr, *dt = nullptr, *dp = nullptr;
+  int rc = DEFMM_OK;
+  auto ck = [&](hipError_t e, const char* what) {
+    if (e != hipSuccess && rc == DEFMM_OK) {
+      g_err = std::string(what) + ": " + hipGetErrorString(e);
+      rc = e == hipErrorOutOfMemory ? DEFMM_EMEMORY : DEFMM_EHIP;
+    }
+  };
+  ck(hipMalloc(&ds, n_src * 24), "hipMalloc");
+  if (!rc) ck(hipMalloc(&dq, n_src * 16), "hipMalloc");
+  if (!rc) ck(hipMalloc(&dt, n_trg * 24), "hipMalloc");
+  if (!rc) ck(hipMalloc(&dp, n_trg * 16), "hipMalloc");
+  if (!rc) ck(hipMemcpy(ds, src_xyz, n_src * 24, hipMemcpyHostToDevice), "hipMemcpy");
+  if (!rc) ck(hipMemcpy(dq, q, n_src * 16, hipMemcpyHostToDevice), "hipMemcpy");
+  if (!rc) ck(hipMemcpy(dt, trg_xyz, n_trg * 24, hipMemcpyHostToDevice), "hipMemcpy");
+  if (!rc) {
+    hipLaunchKernelGGL(dfm::k_direct, dim3((unsigned)((n_trg + 255) / 256)), dim3(256), 0, 0,
+                       (const double*)ds, (int)n_src, (const double2*)dq, (const double*)dt,
+                       (int)n_trg, kappa, (double2*)dp);
+    ck(hipGetLastError(), "k_direct launch");
+  }
+  if (!rc) ck(hipMemcpy(p, dp, n_trg * 16, hipMemcpyDeviceToHost), "hipMemcpy");
+  for (void* m : {ds, dq, dt, dp})
+    if (m) hipFree(m);
+  return rc;
+}
+
+int defmm_plan_build_stats(const double* src_xyz, int64_t n_src, const double* trg_xyz,
+                           int64_t n_trg, int order, int ncrit, double kappa, int lvmax,
+                           int64_t* out, int n) {
+  if (!out) { g_err = "null output pointer"; return DEFMM_EARG; }
+  if (order < 2 || order > dfm::LMAX) { g_err = "order must be in 2..8"; return DEFMM_EARG; }
+  if (lvmax < -1 || (lvmax < 0 && ncrit < 1)) { g_err = "bad ncrit/lvmax"; return DEFMM_EARG; }
+  dfm::Plan* pl = new dfm::Plan();
+  std::string msg;
+  int rc = dfm::buildPlan(*pl, src_xyz, n_src, trg_xyz, n_trg, order, lvmax >= 0 ? -lvmax : ncrit,
+                          kappa, msg);
+  if (rc != dfm::OK) {
+    g_err = msg;
+    delete pl;
+    return rc;
+  }
+  defmm_plan_s tmp;
+  tmp.x.st = pl->st;
+  delete pl;
+  return defmm_plan_stats(&tmp, out, n);
+}
+
+int defmm_plan_destroy(defmm_plan plan) {
+  if (!plan) return DEFMM_OK;
+  hipSetDevice(plan->x.device);
+  hipStreamSynchronize(plan->x.own);
+  delete plan;
+  return DEFMM_OK;
+}
+
+}  // extern "C"

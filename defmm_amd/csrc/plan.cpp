@@ -1,0 +1,878 @@
+// Host plan builder.  Restates the reference's setup (prcmpt,
+// include/interface.hpp:100-151) as list construction: instead of marking
+// cells and later re-walking the trees for every evaluation, one dual walk
+// records every M2L and P2P pair, and the per-stage lists are flattened into
+// CSR arrays that the device executes level by level.
+//
+// Compile without floating-point contraction (-ffp-contract=off): the octree
+// split tests, the admissibility test and the best-cone search are discrete
+// decisions taken on rounded doubles.
+#include "plan.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <tuple>
+#include <unordered_map>
+
+namespace dfm {
+
+static const cplx I1(0., 1.);
+static const double PI4 = 4. * M_PI;
+
+static inline double nrm3(const double v[3]) {
+  double t = 0.;
+  for (int k = 0; k < 3; k++) t += v[k] * v[k];
+  return std::sqrt(t);
+}
+
+// Equispaced Lagrange basis on [-alpha, alpha] (thetis_polynomials.hpp:81-94).
+static double lagrange(int i, double x, int n, double alpha) {
+  double res = 1., nx = x / alpha;
+  double pi = -1. + 2. * ((double)i / (double)(n - 1));
+  for (int j = 0; j < n; j++) {
+    if (j == i) continue;
+    double pj = -1. + 2. * ((double)j / (double)(n - 1));
+    res *= (nx - pj) / (pi - pj);
+  }
+  return res;
+}
+static inline double gridPoint(int i, int n, double alpha) {
+  return alpha * (-1. + 2. * ((double)i / (double)(n - 1)));
+}
+
+int Tree::slotOf(int c, int d) const {
+  if (nslot[c] == 0) return -1;
+  if (slotDir[slot0[c]] < 0) return slot0[c];
+  if (!slotIdx.empty()) return (d >= 0 && mark0[c] >= 0) ? slotIdx[mark0[c] + d] : -1;
+  // directions of a cell's slots are ascending
+  const int* b = slotDir.data() + slot0[c];
+  const int* e = b + nslot[c];
+  const int* it = std::lower_bound(b, e, d);
+  if (it == e || *it != d) return -1;
+  return slot0[c] + (int)(it - b);
+}
+
+// ------------------------------------------------------------- octree ----
+// Stable octant partition with the split test x_k > c_k, children in octant
+// order 0..7 (bit k = axis k), a cell is a leaf when it holds fewer than
+// Ncrit points (sort.hpp:49,98-162,187-220) or, with a fixed depth, when it
+// reaches that level (sort.hpp:51).  Cells are numbered in pre-order.
+namespace {
+struct Builder {
+  Tree& t;
+  int ncrit, lvmax;
+  std::vector<int> tmp;
+  std::vector<double> px, py, pz;
+  int err = OK;
+  Builder(Tree& tr, int nc, int lm) : t(tr), ncrit(nc), lvmax(lm) {}
+
+  void cell(int b, int n, const double ctr[3], double rad, const int idx[3], int lvl) {
+    if (err) return;
+    if (lvl > 30) { err = ERR_DEPTH; return; }
+    int me = (int)t.lvl.size();
+    t.lvl.push_back(lvl);
+    t.nprt.push_back(n);
+    t.pbeg.push_back(b);
+    t.son0.push_back(0);
+    t.nson.push_back(0);
+    for (int k = 0; k < 3; k++) { t.idx.push_back(idx[k]); t.ctr.push_back(ctr[k]); }
+    t.rad.push_back(rad);
+    if ((int)t.nPerLevel.size() <= lvl) t.nPerLevel.push_back(0);
+    t.nPerLevel[lvl]++;
+    bool stop = lvmax < 0 ? (n < ncrit) : (lvl >= lvmax);
+    if (stop) return;
+    int histo[8] = {0}, off[8], add[8] = {0};
+    std::vector<unsigned char> code(n);
+    for (int i = 0; i < n; i++) {
+      int p = t.perm[b + i];
+      int bc = (px[p] > ctr[0] ? 1 : 0) | (py[p] > ctr[1] ? 2 : 0) | (pz[p] > ctr[2] ? 4 : 0);
+      code[i] = (unsigned char)bc;
+      histo[bc]++;
+    }
+    off[0] = 0;
+    for (int i = 1; i < 8; i++) off[i] = off[i - 1] + histo[i - 1];
+    for (int i = 0; i < n; i++) tmp[b + off[code[i]] + (add[code[i]]++)] = t.perm[b + i];
+    std::copy(tmp.begin() + b, tmp.begin() + b + n, t.perm.begin() + b);
+    double nr = rad / 2.;
+    std::vector<int> kids;
+    for (int i = 0; i < 8; i++) {
+      if (!histo[i]) continue;
+      double nc[3];
+      int ni[3];
+      for (int k = 0; k < 3; k++) {
+        nc[k] = ctr[k] + nr * (((i >> k) & 1) * 2 - 1);
+        ni[k] = 2 * idx[k] + ((i >> k) & 1);
+      }
+      kids.push_back((int)t.lvl.size());
+      cell(b + off[i], histo[i], nc, nr, ni, lvl + 1);
+      if (err) return;
+    }
+    // sons are stored after recursion: pre-order ids are known only then
+    t.son0[me] = (int)t.sons.size();
+    t.nson[me] = (int)kids.size();
+    for (int k : kids) t.sons.push_back(k);
+  }
+};
+}  // namespace
+
+static int buildTree(Tree& t, const double* xyz, int64_t n, const double ctr[3], double rad,
+                     int ncrit, int lvmax) {
+  Builder bd(t, ncrit, lvmax);
+  bd.px.resize(n); bd.py.resize(n); bd.pz.resize(n);
+  for (int64_t i = 0; i < n; i++) {
+    bd.px[i] = xyz[3 * i]; bd.py[i] = xyz[3 * i + 1]; bd.pz[i] = xyz[3 * i + 2];
+  }
+  t.perm.resize(n);
+  std::iota(t.perm.begin(), t.perm.end(), 0);
+  bd.tmp.resize(n);
+  int idx0[3] = {0, 0, 0};
+  bd.cell(0, (int)n, ctr, rad, idx0, 0);
+  if (bd.err) return bd.err;
+  t.x.resize(n); t.y.resize(n); t.z.resize(n);
+  for (int64_t i = 0; i < n; i++) {
+    int p = t.perm[i];
+    t.x[i] = bd.px[p]; t.y[i] = bd.py[p]; t.z[i] = bd.pz[p];
+  }
+  return OK;
+}
+
+// --------------------------------------------------------- directions ----
+// newdirections.hpp:75-171: stage 0 = +x,-x,+y,-y,+z,-z; stage e >= 1 =
+// [x-face(+), x-face(-), z-face(+), z-face(-), y-face(+), y-face(-)], each
+// face refined from its parent cone by (0, +-r, +-r), r = 2^-(e+1).
+static void makeDirections(std::vector<std::vector<std::array<double, 3>>>& D, int n) {
+  D.assign(n, {});
+  for (int e = 0; e < n; e++) D[e].resize((size_t)6 << (2 * e));
+  for (int k = 0; k < 3; k++) {
+    D[0][2 * k] = {0., 0., 0.};
+    D[0][2 * k][k] = 1.;
+    D[0][2 * k + 1] = {0., 0., 0.};
+    D[0][2 * k + 1][k] = -1.;
+  }
+  for (int e = 1; e < n; e++) {
+    double r = 1. / double(1 << (e + 1));
+    int nprev = 1 << (2 * (e - 1));
+    for (int i = 0; i < nprev; i++)
+      for (int q = 0; q < 4; q++) {
+        std::array<double, 3> d = D[e - 1][i];
+        d[1] += ((q & 1) ? -1. : 1.) * r;
+        d[2] += ((q & 2) ? -1. : 1.) * r;
+        double nn = nrm3(d.data());
+        for (int k = 0; k < 3; k++) d[k] /= nn;
+        D[e][i * 4 + q] = d;
+      }
+    int face = 1 << (2 * e);
+    for (int i = 0; i < face; i++) {
+      D[e][face + i] = D[e][i];
+      D[e][face + i][0] = -D[e][i][0];
+    }
+    for (int j = 1; j < 3; j++)
+      for (int i = 0; i < 2 * face; i++)
+        for (int k = 0; k < 3; k++) D[e][j * 2 * face + i][k] = D[e][i][(k + j) % 3];
+  }
+}
+
+// First direction at minimal distance; mirror-image cones of a symmetric
+// offset are equidistant, so distances within 1e-12 of the best count as ties.
+static int bestCone(const std::vector<std::array<double, 3>>& D, const double dir[3]) {
+  double best = 1.e30;
+  int ind = -1;
+  for (size_t i = 0; i < D.size(); i++) {
+    double d[3] = {D[i][0] - dir[0], D[i][1] - dir[1], D[i][2] - dir[2]};
+    double nn = nrm3(d);
+    if (nn < best - 1.e-12) { best = nn; ind = (int)i; }
+  }
+  return ind;
+}
+
+// --------------------------------------------------------- symmetries ----
+// hypercube.hpp:108-136 and permutations.hpp:87-150,208-232.
+static const int kSigma[8][3] = {{0, 1, 2}, {1, 0, 2}, {0, 1, 2}, {1, 2, 0},
+                                 {0, 2, 1}, {0, 1, 2}, {2, 0, 1}, {2, 1, 0}};
+static inline void fundamental(const int X[3], int R[3], int& hsh) {
+  int a0 = std::abs(X[0]), a1 = std::abs(X[1]), a2 = std::abs(X[2]);
+  int h1 = (a0 < a1 ? 1 : 0) | (a0 < a2 ? 2 : 0) | (a1 < a2 ? 4 : 0);
+  int h0 = (X[0] < 0 ? 1 : 0) | (X[1] < 0 ? 2 : 0) | (X[2] < 0 ? 4 : 0);
+  hsh = h0 + (h1 << 3);
+  for (int k = 0; k < 3; k++) R[k] = std::abs(X[kSigma[h1][k]]);
+}
+
+// ---------------------------------------------------------------- DFT ----
+void dft3(int P, const cplx* in, cplx* out, int sign) {
+  int PP = P * P * P;
+  std::vector<cplx> tw(P * P), a(in, in + PP), b(PP);
+  for (int j = 0; j < P; j++)
+    for (int k = 0; k < P; k++) {
+      double ang = 2. * M_PI * (double)((j * k) % P) / (double)P;
+      tw[j * P + k] = cplx(std::cos(ang), sign * std::sin(ang));
+    }
+  int strides[3] = {P * P, P, 1};
+  for (int ax = 2; ax >= 0; ax--) {
+    int s = strides[ax];
+    for (int i = 0; i < PP; i++) {
+      int k = (i / s) % P;
+      int base = i - k * s;
+      cplx acc = 0.;
+      for (int j = 0; j < P; j++) acc += a[base + j * s] * tw[j * P + k];
+      b[i] = acc;
+    }
+    a.swap(b);
+  }
+  std::copy(a.begin(), a.end(), out);
+}
+
+// ---------------------------------------------------------- the build ----
+namespace {
+struct PairLists {
+  // M2L pairs in walk order: target cell, source cell, table id, hash, cone
+  std::vector<int> mt, ms, mtab, mh, mcone;
+  // P2P pairs in walk order
+  std::vector<int> pt, ps;
+};
+
+struct Walker {
+  Plan& pl;
+  PairLists& out;
+  // table ids keyed by (level, fundamental offset)
+  std::unordered_map<uint64_t, int> tabId;
+  std::vector<std::array<int, 4>> tabKey;  // level, a, b, c
+  std::unordered_map<uint64_t, int> cone;  // (table id, hash) -> cone
+  // per-level dense cache of offsets met so far: packed table id, hash, cone
+  std::vector<std::vector<uint64_t>> win;
+  std::vector<int> winW;
+  int err = OK;
+  Walker(Plan& p, PairLists& o) : pl(p), out(o) {
+    // the offsets of M2L pairs at a level are bounded by the parents'
+    // non-admissible range; a window of that half-width holds them all
+    win.resize(pl.nlvl);
+    winW.assign(pl.nlvl, -1);
+    for (int l = 0; l < pl.nlvl; l++) {
+      if (!pl.precomp[l]) continue;
+      int W = (1 << std::min(l, 20));
+      if (l > pl.maxStage) {
+        int pv = l - 1;
+        double tt = 2. * (pl.boxRad / double(1 << pv));
+        double reach = (pl.isHF[pv] && pv) ? std::max(tt * tt * pl.kappa, tt) / tt + 1. : 1.;
+        int bp = (int)std::ceil(reach * 1.01) + 1;
+        W = std::min(W, 2 * bp + 2);
+      }
+      size_t n = (size_t)(2 * W + 1) * (2 * W + 1) * (2 * W + 1);
+      if (n > ((size_t)1 << 26)) continue;
+      winW[l] = W;
+      win[l].assign(n, 0);
+    }
+  }
+
+  static uint64_t key4(int l, int a, int b, int c) {
+    return ((uint64_t)l << 48) | ((uint64_t)(uint32_t)a << 32) | ((uint64_t)(uint32_t)b << 16) |
+           (uint64_t)(uint32_t)c;
+  }
+  int table(int l, const int nI[3]) {
+    uint64_t k = key4(l, nI[0], nI[1], nI[2]);
+    auto it = tabId.find(k);
+    if (it != tabId.end()) return it->second;
+    int id = (int)tabKey.size();
+    tabId[k] = id;
+    tabKey.push_back({l, nI[0], nI[1], nI[2]});
+    return id;
+  }
+  // hrzpass.hpp:51-56
+  bool macHF(int i, int j) const {
+    const Tree &T = pl.T, &S = pl.S;
+    double d[3];
+    for (int k = 0; k < 3; k++) d[k] = T.ctr[3 * i + k] - S.ctr[3 * j + k];
+    double tmp = (1. - 1.e-8) * (nrm3(d) - 2. * S.rad[j]);
+    double tt = 2. * S.rad[j];
+    return tmp >= std::max(tt * tt * pl.kappa, tt);
+  }
+  // hrzpass.hpp:39-49
+  bool macLF(int i, int j) const {
+    int test = 0, ncmp = 3;
+    for (int k = 0; k < 3; k++) {
+      int d = pl.T.idx[3 * i + k] - pl.S.idx[3 * j + k];
+      ncmp -= (d ? 0 : 1);
+      test += std::abs(d);
+    }
+    return test > ncmp;
+  }
+  void m2l(int i, int j, bool hf) {
+    int lvl = pl.T.lvl[i];
+    int iZ[3], nI[3], h;
+    for (int k = 0; k < 3; k++) iZ[k] = pl.T.idx[3 * i + k] - pl.S.idx[3 * j + k];
+    uint64_t* slotw = nullptr;
+    int W = winW[lvl];
+    if (W >= 0 && std::abs(iZ[0]) <= W && std::abs(iZ[1]) <= W && std::abs(iZ[2]) <= W) {
+      size_t n1 = 2 * W + 1;
+      slotw = &win[lvl][((size_t)(iZ[0] + W) * n1 + (size_t)(iZ[1] + W)) * n1 + (size_t)(iZ[2] + W)];
+      if (*slotw) {
+        uint64_t v = *slotw;
+        int id = (int)(v & 0xFFFFFFFFu), hh = (int)((v >> 32) & 63), cn = (int)(v >> 40) - 1;
+        if (hf) {
+          pl.T.marks[pl.T.mark0[i] + cn] = 1;
+          pl.S.marks[pl.S.mark0[j] + cn] = 1;
+          pl.st.hf_m2l++;
+        } else {
+          pl.st.lf_m2l++;
+        }
+        out.mt.push_back(i); out.ms.push_back(j); out.mtab.push_back(id);
+        out.mh.push_back(hh); out.mcone.push_back(cn);
+        return;
+      }
+    }
+    fundamental(iZ, nI, h);
+    if (!hf)
+      for (int k = 0; k < 3; k++)
+        if (nI[k] >= pl.nCellPerDirMax[lvl]) err = ERR_TABLE;
+    int id = table(lvl, nI);
+    int cn = -1;
+    if (hf) {
+      // M2L.hpp:189-197: cone of the first pair met with this offset
+      uint64_t ck = ((uint64_t)id << 8) | (uint64_t)h;
+      auto it = cone.find(ck);
+      if (it == cone.end()) {
+        double fZ[3];
+        for (int k = 0; k < 3; k++) fZ[k] = pl.T.ctr[3 * i + k] - pl.S.ctr[3 * j + k];
+        double nn = nrm3(fZ);
+        for (int k = 0; k < 3; k++) fZ[k] /= nn;
+        cn = bestCone(pl.dirs[pl.HFStage[lvl]], fZ);
+        cone[ck] = cn;
+      } else cn = it->second;
+      pl.T.marks[pl.T.mark0[i] + cn] = 1;
+      pl.S.marks[pl.S.mark0[j] + cn] = 1;
+      pl.st.hf_m2l++;
+    } else {
+      pl.st.lf_m2l++;
+    }
+    out.mt.push_back(i); out.ms.push_back(j); out.mtab.push_back(id);
+    out.mh.push_back(h); out.mcone.push_back(cn);
+    if (slotw) *slotw = (uint64_t)(uint32_t)id | ((uint64_t)h << 32) | ((uint64_t)(cn + 1) << 40);
+  }
+  // hrzpass.hpp:61-91 (and the setup walk hrzpass.hpp:111-138, which visits
+  // the HF pairs in the same order)
+  void walk(int i, int j) {
+    if (err) return;
+    const Tree &T = pl.T, &S = pl.S;
+    int l = T.lvl[i];
+    if (pl.precomp[l] && l == S.lvl[j]) {
+      if (l && pl.isHF[l]) {
+        if (macHF(i, j)) { m2l(i, j, true); return; }
+      } else if (macLF(i, j)) {
+        m2l(i, j, false);
+        return;
+      }
+    }
+    if (T.leaf(i) || S.leaf(j)) {
+      out.pt.push_back(i);
+      out.ps.push_back(j);
+      pl.st.p2p_calls++;
+      pl.st.p2p_pairs += (int64_t)T.nprt[i] * S.nprt[j];
+      return;
+    }
+    for (int a = 0; a < T.nson[i]; a++)
+      for (int b = 0; b < S.nson[j]; b++) walk(T.sons[T.son0[i] + a], S.sons[S.son0[j] + b]);
+  }
+};
+}  // namespace
+
+// dwnpass.hpp:80-104: a child needs direction d when its parent has any of
+// the refined directions 4d..4d+3.
+static void propagate(Plan& pl, Tree& t) {
+  for (int c = 0; c < t.ncells(); c++) {
+    int l = t.lvl[c];
+    if (t.leaf(c) || !(pl.isHF[l] && pl.precomp[l])) continue;
+    if (!pl.isHF[l + 1]) continue;
+    for (int a = 0; a < t.nson[c]; a++) {
+      int s = t.sons[t.son0[c] + a];
+      for (int dt = 0; dt < pl.nDir[l + 1]; dt++)
+        for (int k = 0; k < 4; k++)
+          if (t.marks[t.mark0[c] + 4 * dt + k]) t.marks[t.mark0[s] + dt] = 1;
+    }
+  }
+}
+
+// initdevs.hpp:187-245: HF cells get one slot per marked direction, in
+// ascending direction order; LF cells at precomputation levels one slot.
+static void numberSlots(Plan& pl, Tree& t) {
+  int n = t.ncells();
+  t.slot0.assign(n, 0);
+  t.nslot.assign(n, 0);
+  t.slotCell.clear();
+  t.slotDir.clear();
+  for (int c = 0; c < n; c++) {
+    int l = t.lvl[c];
+    t.slot0[c] = (int)t.slotCell.size();
+    if (!pl.precomp[l]) continue;
+    if (pl.isHF[l]) {
+      for (int d = 0; d < pl.nDir[l]; d++)
+        if (t.marks[t.mark0[c] + d]) { t.slotCell.push_back(c); t.slotDir.push_back(d); }
+    } else {
+      t.slotCell.push_back(c);
+      t.slotDir.push_back(-1);
+    }
+    t.nslot[c] = (int)t.slotCell.size() - t.slot0[c];
+  }
+  t.nslots = (int)t.slotCell.size();
+  t.slotIdx.assign(t.marks.size(), -1);
+  for (int s = 0; s < t.nslots; s++) {
+    int c = t.slotCell[s];
+    if (t.slotDir[s] >= 0) t.slotIdx[t.mark0[c] + t.slotDir[s]] = s;
+  }
+}
+
+int validatePlan(const Plan& pl, std::string& msg);
+
+int buildPlan(Plan& pl, const double* src, int64_t ns, const double* trg, int64_t nt, int L,
+              int ncrit, double kappa, std::string& msg) {
+  const int lvmax = ncrit < 0 ? -ncrit : -1;  // negative Ncrit: fixed depth
+  if (!src || !trg || ns <= 0 || nt <= 0 || L < 2 || L > 12 || (ncrit == 0) || kappa <= 0 ||
+      ns > (int64_t)2000000000 || nt > (int64_t)2000000000) {
+    msg = "invalid argument";
+    return ERR_ARG;
+  }
+  pl = Plan();
+  pl.L = L; pl.P = 2 * L - 1; pl.LL = L * L * L; pl.PP = pl.P * pl.P * pl.P;
+  pl.ncrit = ncrit; pl.kappa = kappa;
+  const int LL = pl.LL, PP = pl.PP, P = pl.P;
+
+  // particles.hpp:79-100: one cubic box around both clouds
+  double mn[3], mx[3];
+  for (int k = 0; k < 3; k++) mn[k] = mx[k] = trg[k];
+  auto upd = [&](const double* p, int64_t n) {
+    for (int64_t i = 0; i < n; i++)
+      for (int k = 0; k < 3; k++) {
+        double v = p[3 * i + k];
+        mn[k] = v < mn[k] ? v : mn[k];
+        mx[k] = v > mx[k] ? v : mx[k];
+      }
+  };
+  upd(trg, nt);
+  upd(src, ns);
+  double r1 = 0., r2 = 0.;
+  for (int k = 0; k < 3; k++) pl.boxCtr[k] = (mn[k] + mx[k]) / 2;
+  r1 = pl.boxCtr[0] - mn[0]; r2 = mx[0] - pl.boxCtr[0];
+  for (int k = 1; k < 3; k++) {
+    r1 = std::max(r1, pl.boxCtr[k] - mn[k]);
+    r2 = std::max(r2, mx[k] - pl.boxCtr[k]);
+  }
+  pl.boxRad = std::max(r1, r2);
+  if (!(pl.boxRad > 0.)) { msg = "all points coincide"; return ERR_ARG; }
+
+  int e = buildTree(pl.T, trg, nt, pl.boxCtr, pl.boxRad, ncrit, lvmax);
+  if (!e) e = buildTree(pl.S, src, ns, pl.boxCtr, pl.boxRad, ncrit, lvmax);
+  if (e) { msg = "octree deeper than 30 levels (too many coincident points for Ncrit)"; return e; }
+
+  // frequency.hpp:43-129, 158-184 (eta = 1)
+  int nlvl = std::max((int)pl.T.nPerLevel.size(), (int)pl.S.nPerLevel.size());
+  pl.nlvl = nlvl;
+  pl.isHF.assign(nlvl, 0); pl.HFStage.assign(nlvl, 0); pl.nDir.assign(nlvl, 1);
+  pl.nCellPerDirMax.assign(nlvl, -1); pl.precomp.assign(nlvl, 0);
+  double sd = 2. * pl.boxRad;
+  int cpt = 0;
+  while (sd * kappa > 1.) { sd /= 2.; cpt++; }
+  pl.cpt = cpt;
+  pl.noHF = !(cpt > 1);
+  if (cpt) makeDirections(pl.dirs, cpt);
+  for (int l = 0; l < nlvl && l <= cpt; l++) { pl.isHF[l] = cpt - l + 1; pl.HFStage[l] = cpt - l; }
+  for (int l = 0; l < nlvl; l++)
+    pl.nDir[l] = (pl.isHF[l] && l) ? (int)pl.dirs[pl.HFStage[l]].size() : 1;
+  int ms = 0;
+  while (1. < kappa * (2. * pl.boxRad) / double(1 << ms) / double(1 << ms)) {
+    ms++;
+    if (ms >= nlvl) { msg = "no precomputation level: the tree is too shallow for kappa"; return ERR_NO_PRECOMP; }
+  }
+  ms = std::max(ms, 2);
+  if (ms >= nlvl) { msg = "no precomputation level: the tree has fewer than 3 levels"; return ERR_NO_PRECOMP; }
+  pl.maxStage = ms;
+  for (int l = 0; l < nlvl; l++) pl.precomp[l] = (l >= ms);
+  for (int l = 0; l < nlvl; l++)
+    if (pl.precomp[l])
+      pl.nCellPerDirMax[l] = pl.isHF[l] ? (1 << l) : (pl.isHF[l - 1] ? 8 : 4);
+
+  // effective-direction marks
+  for (Tree* t : {&pl.S, &pl.T}) {
+    t->mark0.assign(t->ncells(), -1);
+    int64_t tot = 0;
+    for (int c = 0; c < t->ncells(); c++) {
+      int l = t->lvl[c];
+      if (pl.precomp[l] && pl.isHF[l]) { t->mark0[c] = tot; tot += pl.nDir[l]; }
+    }
+    t->marks.assign(tot, 0);
+  }
+
+  // dual walk
+  PairLists pr;
+  Walker w(pl, pr);
+  w.walk(0, 0);
+  if (w.err) { msg = "M2L offset outside the table range"; return w.err; }
+  if (!pl.noHF) { propagate(pl, pl.S); propagate(pl, pl.T); }
+  numberSlots(pl, pl.S);
+  numberSlots(pl, pl.T);
+  pl.st.src_slots = pl.S.nslots;
+  pl.st.trg_slots = pl.T.nslots;
+  pl.st.nlvl = nlvl; pl.st.cpt = cpt; pl.st.max_stage = ms;
+  pl.st.ncells_t = pl.T.ncells(); pl.st.ncells_s = pl.S.ncells();
+
+  // ---- tables ----
+  pl.embed.resize(LL);
+  for (int i = 0; i < LL; i++) {
+    int a = i / (L * L), b = (i / L) % L, c = i % L;
+    pl.embed[i] = (a * P + b) * P + c;
+  }
+  pl.perm.assign(64 * PP, 0);
+  for (int p0 = 0; p0 < 8; p0++)
+    for (int p1 = 0; p1 < 8; p1++) {
+      int h = p0 + (p1 << 3);
+      for (int pp = 0; pp < PP; pp++) {
+        int tp[3] = {pp / (P * P), (pp / P) % P, pp % P}, tm[3];
+        for (int k = 0; k < 3; k++) {
+          tm[k] = tp[kSigma[p1][k]];
+          if ((p0 >> kSigma[p1][k]) & 1)
+            if (tm[k]) tm[k] = P - tm[k];
+        }
+        pl.perm[h * PP + pp] = (tm[0] * P + tm[1]) * P + tm[2];
+      }
+    }
+  // circulant.hpp:73-93,133-144: kernel samples on the (2L-1)^3 difference grid
+  std::vector<double> tabZ(3 * PP);
+  for (int i = 0; i < PP; i++) {
+    int E[3] = {i / (P * P), (i / P) % P, i % P};
+    for (int k = 0; k < 3; k++) {
+      int ek = E[k] < L ? E[k] : -(P - E[k]);
+      int ip = ek > -1 ? ek : 0, jn = ek > -1 ? 0 : -ek;
+      tabZ[3 * i + k] = gridPoint(ip, L, 1.) - gridPoint(jn, L, 1.);
+    }
+  }
+  pl.ntab = (int)w.tabKey.size();
+  pl.m2lTab.assign((size_t)pl.ntab * PP, 0.);
+#pragma omp parallel
+  {
+    std::vector<cplx> g(PP);
+#pragma omp for schedule(dynamic, 4)
+    for (int id = 0; id < pl.ntab; id++) {
+      int l = w.tabKey[id][0];
+      double rad = pl.boxRad / double(1 << l);
+      double v[3];
+      for (int k = 0; k < 3; k++) v[k] = 2. * double(w.tabKey[id][1 + k]) * rad;
+      for (int i = 0; i < PP; i++) {
+        double u[3];
+        for (int k = 0; k < 3; k++) u[k] = v[k] + tabZ[3 * i + k] * rad;
+        double R = nrm3(u);
+        g[i] = std::exp(I1 * R * kappa) / (PI4 * R);
+      }
+      dft3(P, g.data(), pl.m2lTab.data() + (size_t)id * PP, -1);
+    }
+  }
+  for (int id = 0; id < pl.ntab; id++) {
+    if (pl.isHF[w.tabKey[id][0]]) pl.st.hf_tables++; else pl.st.lf_tables++;
+  }
+  // planewave.hpp:43-69: phase tables of every precomputation HF level
+  pl.sigRow0.assign(nlvl, -1);
+  int rows = 0;
+  for (int l = 0; l < nlvl; l++)
+    if (pl.precomp[l] && pl.isHF[l]) { pl.sigRow0[l] = rows; rows += pl.nDir[l]; }
+  pl.sig.assign((size_t)rows * LL, 0.);
+  pl.sig2.assign((size_t)rows * LL, 0.);
+  for (int l = 0; l < nlvl; l++) {
+    if (pl.sigRow0[l] < 0) continue;
+    double hr = .5 * (pl.boxRad / double(1 << l));
+#pragma omp parallel for schedule(static)
+    for (int d = 0; d < pl.nDir[l]; d++) {
+      const double* dv = pl.dirs[pl.HFStage[l]][d].data();
+      size_t r = (size_t)(pl.sigRow0[l] + d) * LL;
+      for (int i = 0; i < LL; i++) {
+        int I[3] = {i / (L * L), (i / L) % L, i % L};
+        double dp = 0.;
+        for (int k = 0; k < 3; k++) dp += dv[k] * gridPoint(I[k], L, hr);
+        cplx s = std::exp(-I1 * kappa * dp);
+        pl.sig[r + i] = s;
+        pl.sig2[r + i] = s * s;
+      }
+    }
+  }
+  // interpolation.hpp:183-235
+  for (int s = 0; s < 2; s++) {
+    pl.A[s].assign(L * L, 0.);
+    pl.B[s].assign(L * L, 0.);
+    double sh = s ? 1. : -1.;
+    for (int p = 0; p < L; p++)
+      for (int k = 0; k < L; k++) {
+        double v = lagrange(k, gridPoint(p, L, 1.) + sh, L, 2.);
+        pl.A[s][k * L + p] = v;
+        pl.B[s][p * L + k] = v;
+      }
+  }
+
+  // ---- P2M / L2P leaves (upwpass.hpp:38-45, dwnpass.hpp:38-45) ----
+  for (int c = 0; c < pl.S.ncells(); c++)
+    if (pl.S.leaf(c) && pl.precomp[pl.S.lvl[c]]) {
+      pl.p2mCell.push_back(c);
+      if (pl.isHF[pl.S.lvl[c]]) pl.st.hf_leaves_p2m++; else pl.st.lf_leaves_p2m++;
+    }
+  for (int c = 0; c < pl.T.ncells(); c++)
+    if (pl.T.leaf(c) && pl.precomp[pl.T.lvl[c]]) {
+      pl.l2pCell.push_back(c);
+      if (pl.isHF[pl.T.lvl[c]]) pl.st.hf_l2p++; else pl.st.lf_l2p++;
+    }
+
+  // ---- M2M (upwpass.hpp:50-61; M2M.hpp:34-124) ----
+  // Output = parent slot; inputs = the children's slot for direction d/4
+  // (HF parent above HF children), the child's single slot (HF stage-0
+  // parent above LF children, or LF parent).
+  auto phase = [&](const double* d, const double v[3], double sgn) {
+    double dv = 0.;
+    for (int k = 0; k < 3; k++) dv += d[k] * v[k];
+    return std::exp(sgn * I1 * kappa * dv);
+  };
+  for (int l = nlvl - 2; l >= 0; l--) {
+    if (!pl.precomp[l]) continue;
+    TransLevel tl;
+    tl.level = l;
+    tl.hf = pl.isHF[l] ? 1 : 0;
+    tl.beg.push_back(0);
+    const Tree& S = pl.S;
+    for (int c = 0; c < S.ncells(); c++) {
+      if (S.lvl[c] != l || S.leaf(c)) continue;
+      for (int k = 0; k < S.nslot[c]; k++) {
+        int os = S.slot0[c] + k;
+        int d = S.slotDir[os];
+        for (int a = 0; a < S.nson[c]; a++) {
+          int ch = S.sons[S.son0[c] + a];
+          double v[3];
+          for (int kk = 0; kk < 3; kk++) v[kk] = S.ctr[3 * ch + kk] - S.ctr[3 * c + kk];
+          int in;
+          cplx edv = 1.;
+          if (tl.hf) {
+            in = pl.HFStage[l] ? S.slotOf(ch, d / 4) : S.slotOf(ch, -1);
+            edv = phase(pl.dirs[pl.HFStage[l]][d].data(), v, -1.);
+          } else {
+            in = S.slotOf(ch, -1);
+          }
+          if (in < 0) { msg = "internal: missing child slot in M2M"; return ERR_STATE; }
+          int sg = (v[0] < 0. ? 0 : 1) | (v[1] < 0. ? 0 : 2) | (v[2] < 0. ? 0 : 4);
+          tl.in.push_back(in); tl.sgn.push_back(sg); tl.edv.push_back(edv);
+          tl.sigIdx.push_back(tl.hf ? pl.sigRow0[l] + d : -1);
+        }
+        tl.out.push_back(os);
+        tl.beg.push_back((int)tl.in.size());
+      }
+      if (pl.isHF[l]) { if (pl.HFStage[l]) pl.st.hf_m2m += S.nson[c]; else pl.st.lf2hf_m2m += S.nson[c]; }
+      else pl.st.lf_m2m += S.nson[c];
+    }
+    if (!tl.out.empty()) pl.m2m.push_back(std::move(tl));
+  }
+
+  // ---- L2L (dwnpass.hpp:47-58; L2L.hpp:34-127) ----
+  // Output = child slot; inputs = parent slots (directions 4d..4d+3 of an HF
+  // parent above HF children; all parent slots of an HF stage-0 parent above
+  // an LF child; the LF parent's slot).
+  for (int l = 0; l < nlvl - 1; l++) {
+    if (!pl.precomp[l]) continue;
+    TransLevel tl;
+    tl.level = l;
+    tl.hf = pl.isHF[l] ? 1 : 0;
+    tl.beg.push_back(0);
+    const Tree& T = pl.T;
+    for (int c = 0; c < T.ncells(); c++) {
+      if (T.lvl[c] != l || T.leaf(c)) continue;
+      for (int a = 0; a < T.nson[c]; a++) {
+        int ch = T.sons[T.son0[c] + a];
+        double v[3];
+        for (int kk = 0; kk < 3; kk++) v[kk] = T.ctr[3 * ch + kk] - T.ctr[3 * c + kk];
+        int sg = (v[0] < 0. ? 0 : 1) | (v[1] < 0. ? 0 : 2) | (v[2] < 0. ? 0 : 4);
+        for (int k = 0; k < T.nslot[ch]; k++) {
+          int os = T.slot0[ch] + k;
+          int cd = T.slotDir[os];
+          for (int pk = 0; pk < T.nslot[c]; pk++) {
+            int is = T.slot0[c] + pk;
+            int pd = T.slotDir[is];
+            if (tl.hf && pl.HFStage[l] && pd / 4 != cd) continue;
+            cplx edv = 1.;
+            if (tl.hf) edv = phase(pl.dirs[pl.HFStage[l]][pd].data(), v, 1.);
+            tl.in.push_back(is); tl.sgn.push_back(sg); tl.edv.push_back(edv);
+            tl.sigIdx.push_back(tl.hf ? pl.sigRow0[l] + pd : -1);
+          }
+          tl.out.push_back(os);
+          tl.beg.push_back((int)tl.in.size());
+        }
+        if (pl.isHF[l]) { if (pl.HFStage[l]) pl.st.hf_l2l++; else pl.st.hf2lf_l2l++; }
+        else pl.st.lf_l2l++;
+      }
+    }
+    if (!tl.out.empty()) pl.l2l.push_back(std::move(tl));
+  }
+
+  // ---- M2L, CSR by target slot; M2F only for slots read by M2L ----
+  int64_t nm = (int64_t)pr.mt.size();
+  std::vector<int> tslot(nm), sslot(nm);
+  int missing = 0;
+#pragma omp parallel for schedule(static) reduction(+ : missing)
+  for (int64_t e2 = 0; e2 < nm; e2++) {
+    int c = pr.mcone[e2];
+    tslot[e2] = pl.T.slotOf(pr.mt[e2], c);
+    sslot[e2] = pl.S.slotOf(pr.ms[e2], c);
+    if (tslot[e2] < 0 || sslot[e2] < 0) missing++;
+  }
+  if (missing) { msg = "internal: missing M2L slot"; return ERR_STATE; }
+  pl.m2fIndex.assign(pl.S.nslots, -1);
+  for (size_t e2 = 0; e2 < nm; e2++)
+    if (pl.m2fIndex[sslot[e2]] < 0) pl.m2fIndex[sslot[e2]] = 0;
+  for (int s = 0; s < pl.S.nslots; s++)
+    if (pl.m2fIndex[s] == 0) { pl.m2fIndex[s] = (int)pl.m2fSlot.size(); pl.m2fSlot.push_back(s); }
+  pl.m2lBeg.assign(pl.T.nslots + 1, 0);
+  for (size_t e2 = 0; e2 < nm; e2++) pl.m2lBeg[tslot[e2] + 1]++;
+  for (int s = 0; s < pl.T.nslots; s++) pl.m2lBeg[s + 1] += pl.m2lBeg[s];
+  pl.m2lSrcF.resize(nm); pl.m2lTabId.resize(nm); pl.m2lHash.resize(nm);
+  {
+    std::vector<int> fill(pl.m2lBeg.begin(), pl.m2lBeg.end() - 1);
+    for (size_t e2 = 0; e2 < nm; e2++) {
+      int pos = fill[tslot[e2]]++;
+      pl.m2lSrcF[pos] = pl.m2fIndex[sslot[e2]];
+      pl.m2lTabId[pos] = pr.mtab[e2];
+      pl.m2lHash[pos] = pr.mh[e2];
+    }
+  }
+  pl.st.m2f_slots = (int64_t)pl.m2fSlot.size();
+  pl.st.m2l_entries = (int64_t)nm;
+  for (int s = 0; s < pl.T.nslots; s++)
+    if (pl.m2lBeg[s + 1] > pl.m2lBeg[s]) pl.st.m2l_targets++;
+
+  // ---- P2P, CSR by target leaf (P2P.hpp:42-79 over whole cell ranges) ----
+  // A pair with a non-leaf target cell is split over its descendant leaves;
+  // each leaf's source ranges are sorted and touching ranges merged.
+  {
+    const Tree& T = pl.T;
+    std::vector<int> leafOfCell(T.ncells(), -1);
+    std::vector<int> leaves;
+    for (int c = 0; c < T.ncells(); c++)
+      if (T.leaf(c)) { leafOfCell[c] = (int)leaves.size(); leaves.push_back(c); }
+    // descendant leaves of a cell are a contiguous run in pre-order
+    std::vector<int> firstLeaf(T.ncells()), nLeaves(T.ncells());
+    for (int c = T.ncells() - 1; c >= 0; c--) {
+      if (T.leaf(c)) { firstLeaf[c] = leafOfCell[c]; nLeaves[c] = 1; continue; }
+      int f = 1 << 30, n = 0;
+      for (int a = 0; a < T.nson[c]; a++) {
+        int s = T.sons[T.son0[c] + a];
+        f = std::min(f, firstLeaf[s]);
+        n += nLeaves[s];
+      }
+      firstLeaf[c] = f; nLeaves[c] = n;
+    }
+    std::vector<int64_t> cnt(leaves.size() + 1, 0);
+    for (size_t k = 0; k < pr.pt.size(); k++)
+      for (int q = 0; q < nLeaves[pr.pt[k]]; q++) cnt[firstLeaf[pr.pt[k]] + q + 1]++;
+    for (size_t k = 0; k < leaves.size(); k++) cnt[k + 1] += cnt[k];
+    std::vector<int> rb(cnt.back()), re(cnt.back());
+    {
+      std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
+      for (size_t k = 0; k < pr.pt.size(); k++) {
+        int s = pr.ps[k];
+        for (int q = 0; q < nLeaves[pr.pt[k]]; q++) {
+          int64_t pos = fill[firstLeaf[pr.pt[k]] + q]++;
+          rb[pos] = pl.S.pbeg[s];
+          re[pos] = pl.S.pbeg[s] + pl.S.nprt[s];
+        }
+      }
+    }
+    // sort each leaf's ranges and merge touching ones (in parallel), then
+    // concatenate in leaf order
+    int64_t nl = (int64_t)leaves.size();
+    std::vector<int> nOut(nl, 0);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t k = 0; k < nl; k++) {
+      int64_t b = cnt[k], e = cnt[k + 1];
+      if (b == e) continue;
+      std::vector<std::pair<int, int>> rr;
+      rr.reserve(e - b);
+      for (int64_t q = b; q < e; q++) rr.push_back({rb[q], re[q]});
+      std::sort(rr.begin(), rr.end());
+      // merge touching ranges only; overlapping ones stay separate so that
+      // every listed pair is summed exactly as often as the walk lists it
+      int64_t w = b;
+      int cb = rr[0].first, ce = rr[0].second;
+      for (size_t q = 1; q < rr.size(); q++) {
+        if (rr[q].first == ce) { ce = rr[q].second; continue; }
+        rb[w] = cb; re[w] = ce; w++;
+        cb = rr[q].first; ce = rr[q].second;
+      }
+      rb[w] = cb; re[w] = ce; w++;
+      nOut[k] = (int)(w - b);
+    }
+    pl.p2pBeg.push_back(0);
+    for (int64_t k = 0; k < nl; k++) {
+      if (!nOut[k]) continue;
+      for (int64_t q = cnt[k]; q < cnt[k] + nOut[k]; q++) {
+        pl.p2pSrcB.push_back(rb[q]);
+        pl.p2pSrcE.push_back(re[q]);
+      }
+      pl.p2pLeaf.push_back(leaves[k]);
+      pl.p2pBeg.push_back((int)pl.p2pSrcB.size());
+    }
+  }
+  return validatePlan(pl, msg);
+}
+
+// Every index the device kernels follow, checked on the host before upload.
+int validatePlan(const Plan& pl, std::string& msg) {
+  auto bad = [&](const char* what) {
+    msg = std::string("internal: plan index out of range: ") + what;
+    return ERR_STATE;
+  };
+  const int LL = pl.LL;
+  const int nrows = (int)(pl.sig.size() / (size_t)LL);
+  const int64_t ns = (int64_t)pl.S.x.size(), nt = (int64_t)pl.T.x.size();
+  for (const Tree* t : {&pl.S, &pl.T}) {
+    int64_t np = (int64_t)t->x.size();
+    for (int c = 0; c < t->ncells(); c++) {
+      if (t->pbeg[c] < 0 || t->nprt[c] < 1 || (int64_t)t->pbeg[c] + t->nprt[c] > np) return bad("cell range");
+      if (t->slot0[c] < 0 || t->slot0[c] + t->nslot[c] > t->nslots) return bad("cell slots");
+      if (t->lvl[c] < 0 || t->lvl[c] >= pl.nlvl) return bad("cell level");
+      int l = t->lvl[c];
+      for (int k = 0; k < t->nslot[c]; k++) {
+        int d = t->slotDir[t->slot0[c] + k];
+        if (pl.isHF[l] && (d < 0 || d >= pl.nDir[l] || pl.sigRow0[l] < 0)) return bad("slot direction");
+      }
+    }
+  }
+  for (int c : pl.p2mCell)
+    if (c < 0 || c >= pl.S.ncells() || !pl.S.leaf(c)) return bad("p2m leaf");
+  for (int c : pl.l2pCell)
+    if (c < 0 || c >= pl.T.ncells() || !pl.T.leaf(c)) return bad("l2p leaf");
+  for (int w = 0; w < 2; w++) {
+    const auto& lst = w ? pl.l2l : pl.m2m;
+    int nout = w ? pl.T.nslots : pl.S.nslots;
+    for (const auto& tl : lst) {
+      if (tl.beg.size() != tl.out.size() + 1 || tl.beg.back() != (int)tl.in.size()) return bad("trans csr");
+      for (int o : tl.out)
+        if (o < 0 || o >= nout) return bad("trans out");
+      for (size_t e = 0; e < tl.in.size(); e++) {
+        if (tl.in[e] < 0 || tl.in[e] >= nout) return bad("trans in");
+        if (tl.sigIdx[e] >= nrows || (tl.hf && tl.sigIdx[e] < 0)) return bad("trans phase row");
+        if (tl.sgn[e] < 0 || tl.sgn[e] > 7) return bad("trans sign");
+      }
+    }
+  }
+  for (int s : pl.m2fSlot)
+    if (s < 0 || s >= pl.S.nslots) return bad("m2f slot");
+  if ((int)pl.m2lBeg.size() != pl.T.nslots + 1 || pl.m2lBeg.back() != (int)pl.m2lSrcF.size()) return bad("m2l csr");
+  for (size_t e = 0; e < pl.m2lSrcF.size(); e++) {
+    if (pl.m2lSrcF[e] < 0 || pl.m2lSrcF[e] >= (int)pl.m2fSlot.size()) return bad("m2l source");
+    if (pl.m2lTabId[e] < 0 || pl.m2lTabId[e] >= pl.ntab || pl.m2lTabId[e] >= (1 << 25)) return bad("m2l table");
+    if (pl.m2lHash[e] < 0 || pl.m2lHash[e] > 63) return bad("m2l hash");
+  }
+  for (int v : pl.perm)
+    if (v < 0 || v >= pl.PP) return bad("permutation");
+  if (pl.p2pBeg.size() != pl.p2pLeaf.size() + 1) return bad("p2p csr");
+  for (size_t k = 0; k < pl.p2pLeaf.size(); k++) {
+    int c = pl.p2pLeaf[k];
+    if (c < 0 || c >= pl.T.ncells() || !pl.T.leaf(c)) return bad("p2p leaf");
+    if (pl.p2pBeg[k + 1] <= pl.p2pBeg[k]) return bad("p2p empty list");
+  }
+  for (size_t r = 0; r < pl.p2pSrcB.size(); r++)
+    if (pl.p2pSrcB[r] < 0 || pl.p2pSrcE[r] <= pl.p2pSrcB[r] || pl.p2pSrcE[r] > ns) return bad("p2p range");
+  (void)nt;
+  return OK;
+}
+
+}  // namespace dfm

@@ -1,0 +1,132 @@
+// Host-side evaluation plan of the directional FMM: octree, frequency regimes,
+// direction cones, effective-direction slots, flattened per-stage work lists
+// and the operator tables the device kernels read.  Built once per prcmpt on
+// the host (in double precision, strict IEEE: this file must be compiled
+// without floating-point contraction so that the discrete choices are the
+// same on every compiler), then uploaded by the executor.
+#pragma once
+#include <array>
+#include <complex>
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+namespace dfm {
+
+typedef std::complex<double> cplx;
+
+// Error codes returned through the C ABI.
+enum {
+  OK = 0,
+  ERR_ARG = 1,          // invalid argument (null pointer, bad size, bad order)
+  ERR_NO_PRECOMP = 2,   // tree too shallow for any M2L level (frequency.hpp:112-121)
+  ERR_DEPTH = 3,        // octree deeper than 30 levels (>= Ncrit coincident points)
+  ERR_HIP = 4,          // a HIP runtime call failed
+  ERR_MEMORY = 5,       // device or host allocation failed
+  ERR_TABLE = 6,        // an M2L offset outside the precomputed table range
+  ERR_NO_DEVICE = 7,    // no GPU visible
+  ERR_STATE = 8,        // call out of sequence (e.g. execute before create)
+};
+
+// One octree (sources or targets), cells in pre-order.
+struct Tree {
+  std::vector<double> x, y, z;  // points in tree order
+  std::vector<int> perm;        // tree position -> input index
+  std::vector<int> lvl, nprt, pbeg, son0, nson, sons;
+  std::vector<int> idx;         // 3 per cell
+  std::vector<double> ctr;      // 3 per cell
+  std::vector<double> rad;
+  std::vector<int> nPerLevel;
+  // effective directions: marks are per cell over all directions of its level
+  std::vector<int64_t> mark0;   // offset into marks, -1 if the cell has none
+  std::vector<uint8_t> marks;
+  std::vector<int> slotIdx;     // per (cell, direction) of marks: slot or -1
+  // slots: a slot is (cell, effective direction); LF cells have one slot
+  std::vector<int> slot0, nslot;  // per cell
+  std::vector<int> slotCell, slotDir;  // per slot (dir -1 for LF)
+  int nslots = 0;
+  int ncells() const { return (int)lvl.size(); }
+  bool leaf(int c) const { return nson[c] == 0; }
+  // slot of cell c for direction d (HF) or its single slot (LF); -1 if none
+  int slotOf(int c, int d) const;
+};
+
+struct Stats {
+  int64_t p2p_calls = 0, p2p_pairs = 0, hf_m2l = 0, lf_m2l = 0, src_slots = 0, trg_slots = 0;
+  int64_t hf_tables = 0, lf_tables = 0, hf_leaves_p2m = 0, lf_leaves_p2m = 0;
+  int64_t hf_m2m = 0, lf2hf_m2m = 0, lf_m2m = 0, hf_l2p = 0, lf_l2p = 0;
+  int64_t hf_l2l = 0, hf2lf_l2l = 0, lf_l2l = 0, nlvl = 0, cpt = 0, max_stage = 0;
+  int64_t ncells_t = 0, ncells_s = 0;
+  int64_t p2p_items = 0, m2f_slots = 0, m2l_entries = 0, m2l_targets = 0;
+};
+
+// A per-level translation list (M2M or L2L): for every output slot, the
+// contributing input slots.  For M2M the output is a parent source slot and
+// the inputs are child slots; for L2L the output is a child target slot and
+// the inputs are parent slots.
+struct TransLevel {
+  int level = 0;          // level of the parent cells
+  int hf = 0;             // 1: the parent level is HF (phases apply)
+  std::vector<int> out;   // output slot
+  std::vector<int> beg;   // CSR into entries (size out.size()+1)
+  std::vector<int> in;    // input slot
+  std::vector<int> sgn;   // bit k set: v_k >= 0 (selects the '+' 1D matrix)
+  std::vector<int> sigIdx;  // phase table row (parent level, direction), -1 for LF
+  std::vector<cplx> edv;  // scalar phase factor
+};
+
+struct Plan {
+  // parameters
+  int L = 0, P = 0, LL = 0, PP = 0, ncrit = 0;
+  double kappa = 0;
+  int nlvl = 0, cpt = 0, maxStage = 0;
+  bool noHF = true;
+  std::vector<int> isHF, HFStage, nDir, nCellPerDirMax;
+  std::vector<char> precomp;
+  std::vector<std::vector<std::array<double, 3>>> dirs;  // per HF stage
+  double boxCtr[3] = {0, 0, 0}, boxRad = 0;
+  Tree S, T;
+
+  // phase tables: for every (precomputation HF level, direction) row r,
+  // sig[r*LL + I] = exp(-i kappa d.(r_l/2) xi_I), sig2 = sig^2
+  std::vector<int> sigRow0;  // per level: first row (or -1)
+  std::vector<cplx> sig, sig2;
+  // 1D interpolation matrices: A[s][k*L+p] (M2M: parent k <- child p) and
+  // B[s][p*L+k] (L2L: child p <- parent k), s = 0 for negative offset.
+  std::vector<double> A[2], B[2];
+  // M2L kernel tables (Fourier domain) and index permutations
+  std::vector<cplx> m2lTab;     // ntab * PP
+  int ntab = 0;
+  std::vector<int> perm;        // 64 * PP
+  std::vector<int> embed;       // LL: I2i(I, P)
+
+  // ---- work lists ----
+  // P2M / L2P leaves: cell, direction rows
+  std::vector<int> p2mCell;     // source leaves at precomputation levels
+  std::vector<int> l2pCell;     // target leaves at precomputation levels
+  std::vector<TransLevel> m2m;  // bottom-up order
+  std::vector<TransLevel> l2l;  // top-down order
+  std::vector<int> m2fSlot;     // source slots whose F is read by M2L
+  std::vector<int> m2fIndex;    // per source slot: row in the F store, -1 if none
+  // M2L, CSR by target slot (every target slot appears, possibly empty)
+  std::vector<int> m2lBeg;      // nslots(T)+1
+  std::vector<int> m2lSrcF;     // F row of the source slot
+  std::vector<int> m2lTabId;    // kernel table
+  std::vector<int> m2lHash;     // permutation
+  // P2P, CSR by target leaf: source particle ranges [b, e)
+  std::vector<int> p2pLeaf;     // target leaf cells that receive near field
+  std::vector<int> p2pBeg;      // CSR
+  std::vector<int> p2pSrcB, p2pSrcE;
+
+  Stats st;
+};
+
+// Build the plan.  Returns an error code (OK on success); msg gets details.
+int buildPlan(Plan& pl, const double* src, int64_t ns, const double* trg, int64_t nt, int L,
+              int ncrit, double kappa, std::string& msg);
+
+// Separable unnormalised DFT of a P^3 array (sign -1: forward).
+void dft3(int P, const cplx* in, cplx* out, int sign);
+
+}  // namespace dfm

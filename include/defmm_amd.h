@@ -1,0 +1,135 @@
+/*
+ * defmm_amd.h -- C ABI of the MI355X evaluation pass of defmm's directional
+ * equispaced-interpolation FMM for the 3D Helmholtz kernel
+ * exp(i*kappa*r)/(4*pi*r).
+ *
+ * The reference exposes this path as the C++ class defmm::IBFMM_Mat<3>
+ * (include/interface.hpp:45-224 of the reference): particles are added,
+ * prcmpt() builds trees and tables, gemv() runs one evaluation.  Its state is
+ * process-global and only one gemv per prcmpt is valid; here a plan is an
+ * object, owns its device memory, and can be executed any number of times.
+ * Only plain pointers and sizes cross this boundary.  Every entry point
+ * returns 0 on success or a DEFMM_E* code; defmm_last_error() gives a message
+ * (thread-local).  A plan is thread-compatible (one caller at a time).
+ */
+#ifndef DEFMM_AMD_H
+#define DEFMM_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  DEFMM_OK = 0,
+  DEFMM_EARG = 1,        /* invalid argument */
+  DEFMM_ENOPRECOMP = 2,  /* no M2L level (reference: frequency.hpp:112-121 exits) */
+  DEFMM_EDEPTH = 3,      /* octree deeper than 30 levels (>= Ncrit coincident points) */
+  DEFMM_EHIP = 4,        /* HIP runtime failure */
+  DEFMM_EMEMORY = 5,     /* allocation failure */
+  DEFMM_ETABLE = 6,      /* M2L offset outside the precomputed tables */
+  DEFMM_ENODEVICE = 7,   /* no GPU visible */
+  DEFMM_ESTATE = 8       /* call out of sequence */
+};
+
+/* Arithmetic of an execution. */
+enum {
+  DEFMM_FP64 = 0,  /* complex128 everywhere: the drop-in of the reference */
+  DEFMM_MIXED = 1  /* complex64 expansions and near field, float64 phase
+                      arguments and kernel tables built in float64 */
+};
+
+typedef struct defmm_plan_s* defmm_plan;
+
+/*
+ * Setup: replaces IBFMM_Mat::add{Source,Target}Particle* + prcmpt(L, Ncrit,
+ * kappa, lvMAX) (reference include/interface.hpp:59-151).  src_xyz / trg_xyz
+ * hold n*3 doubles (x, y, z per point, input order).  order = L (>= 2),
+ * ncrit: a cell is split while it holds >= ncrit points; lvmax >= 0 instead
+ * splits every cell down to depth lvmax (the reference's fixed-depth build,
+ * tree.hpp:104-132), -1 to use ncrit.  precision: DEFMM_FP64 or DEFMM_MIXED
+ * (device storage is allocated for that arithmetic).  Device: the current
+ * HIP device.
+ */
+int defmm_plan_create(const double* src_xyz, int64_t n_src, const double* trg_xyz,
+                      int64_t n_trg, int order, int ncrit, double kappa, int lvmax,
+                      int precision, defmm_plan* out);
+
+/*
+ * Evaluation: replaces gemv(A, In, Out) (reference include/interface.hpp:
+ * 153-170).  q: n_src complex charges (interleaved re, im doubles, input
+ * order, host memory); p: n_trg complex potentials, overwritten (host).
+ * Synchronous.
+ */
+int defmm_execute(defmm_plan plan, const double* q, double* p);
+
+/*
+ * Same with device-resident data: q_dev / p_dev point to HIP device memory
+ * holding interleaved complex128 values in input order.  stream: a
+ * hipStream_t (NULL = default stream); the call enqueues work and returns.
+ */
+int defmm_execute_device(defmm_plan plan, const void* q_dev, void* p_dev, void* stream);
+
+/*
+ * Evaluation of a part of the interaction, for verification: parts bit 0 =
+ * near field (P2P), bit 1 = far field (P2M ... L2P).  parts = 3 is
+ * defmm_execute.  Host memory, synchronous.
+ */
+int defmm_execute_parts(defmm_plan plan, const double* q, double* p, int parts);
+
+/*
+ * Plan statistics (list sizes and call counts of the reference's passes);
+ * writes min(n, DEFMM_NSTATS) int64 values, see DEFMM_STAT_* below.
+ */
+int defmm_plan_stats(defmm_plan plan, int64_t* out, int n);
+
+/*
+ * Milliseconds spent in each stage by the last execution (HIP events on the
+ * execution stream): writes min(n, DEFMM_NSTAGES) doubles in the order
+ * gather, P2M, M2M, M2F, M2L+F2L, L2L, L2P, P2P, scatter.
+ */
+int defmm_stage_times(defmm_plan plan, double* ms, int n);
+
+/*
+ * Host-only setup, no device work: builds the octrees and lists of
+ * defmm_plan_create and writes their statistics (as defmm_plan_stats).  For
+ * inspecting list sizes before committing device memory.
+ */
+int defmm_plan_build_stats(const double* src_xyz, int64_t n_src, const double* trg_xyz,
+                           int64_t n_trg, int order, int ncrit, double kappa, int lvmax,
+                           int64_t* out, int n);
+
+/*
+ * FP64 direct sum p_i = sum_j exp(i kappa r_ij)/(4 pi r_ij) q_j (terms with
+ * r_ij^2 < 1e-16 skipped) on the current device, compensated accumulation:
+ * the check of the reference's CheckFULL (include/interface.hpp:178-221),
+ * for sampled targets.  Host arrays, synchronous; allocates device memory.
+ */
+int defmm_direct(const double* src_xyz, int64_t n_src, const double* q, const double* trg_xyz,
+                 int64_t n_trg, double kappa, double* p);
+
+/* Bytes of device memory held by the plan. */
+int defmm_plan_device_bytes(defmm_plan plan, int64_t* bytes);
+
+int defmm_plan_destroy(defmm_plan plan);
+const char* defmm_strerror(int code);
+const char* defmm_last_error(void);
+
+#define DEFMM_NSTAGES 9
+
+enum {
+  DEFMM_STAT_P2P_CALLS = 0, DEFMM_STAT_P2P_PAIRS, DEFMM_STAT_HF_M2L, DEFMM_STAT_LF_M2L,
+  DEFMM_STAT_SRC_SLOTS, DEFMM_STAT_TRG_SLOTS, DEFMM_STAT_HF_TABLES, DEFMM_STAT_LF_TABLES,
+  DEFMM_STAT_HF_LEAVES, DEFMM_STAT_LF_LEAVES, DEFMM_STAT_HF_M2M, DEFMM_STAT_LF2HF_M2M,
+  DEFMM_STAT_LF_M2M, DEFMM_STAT_HF_L2P, DEFMM_STAT_LF_L2P, DEFMM_STAT_HF_L2L,
+  DEFMM_STAT_HF2LF_L2L, DEFMM_STAT_LF_L2L, DEFMM_STAT_NLVL, DEFMM_STAT_CPT,
+  DEFMM_STAT_MAX_STAGE, DEFMM_STAT_NCELLS_T, DEFMM_STAT_NCELLS_S, DEFMM_STAT_P2P_ITEMS,
+  DEFMM_STAT_M2F_SLOTS, DEFMM_STAT_M2L_ENTRIES, DEFMM_STAT_M2L_TARGETS,
+  DEFMM_NSTATS
+};
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DEFMM_AMD_H */
