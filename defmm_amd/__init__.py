@@ -72,6 +72,7 @@ def lib():
     L.defmm_plan_build_stats.argtypes = [D, ctypes.c_int64, D, ctypes.c_int64, ctypes.c_int,
                                          ctypes.c_int, ctypes.c_double, ctypes.c_int, I64,
                                          ctypes.c_int]
+    L.defmm_plan_set_precision.argtypes = [P, ctypes.c_int]
     L.defmm_direct.argtypes = [D, ctypes.c_int64, D, D, ctypes.c_int64, ctypes.c_double, D]
     _lib = L
     return L
@@ -200,6 +201,12 @@ class IBFMM_Mat:
         self._need_plan()
         _check(lib().defmm_execute_device(self._plan, ctypes.c_void_p(q_ptr),
                                           ctypes.c_void_p(p_ptr), ctypes.c_void_p(stream or 0)))
+
+    def set_precision(self, precision):
+        """Switch an existing plan to FP64 or MIXED (lists are kept)."""
+        self._need_plan()
+        _check(lib().defmm_plan_set_precision(self._plan, int(precision)))
+        self.precision = precision
 
     def stats(self):
         self._need_plan()

@@ -769,8 +769,11 @@ struct Exec {
   std::vector<void*> mem;
   int64_t bytes = 0;
   hipStream_t own = nullptr;
+  hipStream_t side = nullptr;     // near field runs here, overlapping the far field
   hipEvent_t ev[DEFMM_NSTAGES + 1] = {};
+  hipEvent_t evFork = nullptr, evJoin = nullptr, evP2P0 = nullptr, evP2P1 = nullptr;
   bool evRecorded = false;
+  bool p2pRecorded = false;
 
   double *sx = nullptr, *sy = nullptr, *sz = nullptr, *tx = nullptr, *ty = nullptr, *tz = nullptr;
   int *sperm = nullptr, *tperm = nullptr;
@@ -795,12 +798,29 @@ struct Exec {
   void *q = nullptr, *p = nullptr, *ML = nullptr, *F = nullptr;
   double2 *qio = nullptr, *pio = nullptr;
   double stageMs[DEFMM_NSTAGES] = {};
+  // host copies needed to re-upload the typed buffers in another precision
+  std::vector<cplx> hSig, hSig2, hTab;
+  std::vector<double> hA[2], hB[2];
+  size_t nMLslots = 0;
+  std::vector<void*> typedMem;
+  int64_t typedBytes = 0;
+  bool typedPhase = false;  // allocations go to typedMem
 
+  void freeTyped() {
+    for (void* m : typedMem) hipFree(m);
+    typedMem.clear();
+    bytes -= typedBytes;
+    typedBytes = 0;
+  }
   ~Exec() {
+    for (void* m : typedMem) hipFree(m);
     for (void* m : mem) hipFree(m);
     for (auto& e : ev)
       if (e) hipEventDestroy(e);
+    for (auto e : {evFork, evJoin, evP2P0, evP2P1})
+      if (e) hipEventDestroy(e);
     if (own) hipStreamDestroy(own);
+    if (side) hipStreamDestroy(side);
   }
 
   template <class X>
@@ -809,8 +829,9 @@ struct Exec {
     if (n == 0) n = 1;
     void* d = nullptr;
     HIPCK(hipMalloc(&d, n * sizeof(X)));
-    mem.push_back(d);
+    (typedPhase ? typedMem : mem).push_back(d);
     bytes += (int64_t)(n * sizeof(X));
+    if (typedPhase) typedBytes += (int64_t)(n * sizeof(X));
     dst = (X*)d;
     if (src) HIPCK(hipMemcpy(d, src, n * sizeof(X), hipMemcpyHostToDevice));
     return OK;
@@ -823,8 +844,9 @@ struct Exec {
     void* d = nullptr;
     if (nbytes == 0) nbytes = 16;
     HIPCK(hipMalloc(&d, nbytes));
-    mem.push_back(d);
+    (typedPhase ? typedMem : mem).push_back(d);
     bytes += (int64_t)nbytes;
+    if (typedPhase) typedBytes += (int64_t)nbytes;
     dst = d;
     return OK;
   }
@@ -847,18 +869,19 @@ static std::vector<typename C2T<R>::T> toR2(const std::vector<cplx>& v) {
   } while (0)
 
 template <typename R>
-static int uploadTyped(Exec& X, Plan& pl) {
+static int uploadTyped(Exec& X) {
   typedef typename C2T<R>::T R2;
-  const int L = pl.L, P = pl.P;
+  const int L = X.L, P = X.P;
+  X.typedPhase = true;
   {
-    auto a = toR2<R>(pl.sig);
+    auto a = toR2<R>(X.hSig);
     R2* d;
     CKR(X.upv(d, a));
     X.sig = d;
-    a = toR2<R>(pl.sig2);
+    a = toR2<R>(X.hSig2);
     CKR(X.upv(d, a));
     X.sig2 = d;
-    a = toR2<R>(pl.m2lTab);
+    a = toR2<R>(X.hTab);
     CKR(X.upv(d, a));
     X.U = d;
   }
@@ -875,21 +898,28 @@ static int uploadTyped(Exec& X, Plan& pl) {
     for (double v : xi) c.push_back((R)v);
     for (double v : iden) c.push_back((R)v);
     for (int s = 0; s < 2; s++)
-      for (double v : pl.A[s]) c.push_back((R)v);
+      for (double v : X.hA[s]) c.push_back((R)v);
     for (int s = 0; s < 2; s++)
-      for (double v : pl.B[s]) c.push_back((R)v);
+      for (double v : X.hB[s]) c.push_back((R)v);
     for (int m = 0; m < P; m++) c.push_back((R)std::cos(2. * M_PI * (double)m / (double)P));
     for (int m = 0; m < P; m++) c.push_back((R)std::sin(2. * M_PI * (double)m / (double)P));
     R* d;
     CKR(X.upv(d, c));
     X.consts = d;
   }
-  size_t slots = (size_t)std::max(pl.S.nslots, pl.T.nslots);
-  CKR(X.alloc(X.ML, slots * pl.LL * sizeof(R2)));
-  CKR(X.alloc(X.F, (size_t)pl.m2fSlot.size() * pl.PP * sizeof(R2)));
+  CKR(X.alloc(X.ML, X.nMLslots * X.LL * sizeof(R2)));
+  CKR(X.alloc(X.F, (size_t)X.nm2f * X.PP * sizeof(R2)));
   CKR(X.alloc(X.q, (size_t)X.ns * sizeof(R2)));
   CKR(X.alloc(X.p, (size_t)X.nt * sizeof(R2)));
+  X.typedPhase = false;
   return OK;
+}
+
+static int uploadPrecision(Exec& X) {
+  X.freeTyped();
+  int rc = X.prec == DEFMM_FP64 ? uploadTyped<double>(X) : uploadTyped<float>(X);
+  X.typedPhase = false;
+  return rc;
 }
 
 static int upload(Exec& X, Plan& pl) {
@@ -1022,8 +1052,12 @@ static int upload(Exec& X, Plan& pl) {
     CKR(X.alloc(d, (size_t)X.nt * sizeof(double2)));
     X.pio = (double2*)d;
   }
-  if (X.prec == DEFMM_FP64) return uploadTyped<double>(X, pl);
-  return uploadTyped<float>(X, pl);
+  X.hSig.swap(pl.sig);
+  X.hSig2.swap(pl.sig2);
+  X.hTab.swap(pl.m2lTab);
+  for (int s = 0; s < 2; s++) { X.hA[s] = pl.A[s]; X.hB[s] = pl.B[s]; }
+  X.nMLslots = (size_t)std::max(pl.S.nslots, pl.T.nslots);
+  return uploadPrecision(X);
 }
 
 static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
@@ -1054,6 +1088,18 @@ static int runTyped(Exec& X, const double2* qin, double2* pout, int parts, hipSt
   hipLaunchKernelGGL(k_gather<R>, dim3(nblk(X.ns, 256)), dim3(256), 0, s, qin, X.sperm, q, (int)X.ns);
   mark(1);
   const bool far = (parts & 2) != 0, near = (parts & 1) != 0;
+  if (near && X.nitems) {
+    // near field (P2P) on the side stream: VALU-bound, it overlaps the
+    // memory-bound far field; it only writes the per-item partial sums
+    HIPCK(hipEventRecord(X.evFork, s));
+    HIPCK(hipStreamWaitEvent(X.side, X.evFork, 0));
+    HIPCK(hipEventRecord(X.evP2P0, X.side));
+    hipLaunchKernelGGL(k_p2p<R>, dim3(X.nitems), dim3(64), 0, X.side, X.items, X.leafCell,
+                       X.T.pbeg, X.T.nprt, X.tx, X.ty, X.tz, X.sx, X.sy, X.sz, (const R2*)q, X.rb,
+                       X.re, X.kappa, X.partial);
+    HIPCK(hipEventRecord(X.evP2P1, X.side));
+    HIPCK(hipEventRecord(X.evJoin, X.side));
+  }
   if (far && X.np2m)
     hipLaunchKernelGGL((k_p2m<R, L>), dim3(X.np2m), dim3(128), 0, s, X.p2m, X.S, X.lv, X.sx, X.sy,
                        X.sz, (const R2*)q, X.dirs, sig2, oc, X.kappa, ML);
@@ -1082,9 +1128,9 @@ static int runTyped(Exec& X, const double2* qin, double2* pout, int parts, hipSt
                        X.tz, X.dirs, sig2, (const R2*)ML, oc, X.kappa, p);
   mark(7);
   if (near && X.nitems) {
-    hipLaunchKernelGGL(k_p2p<R>, dim3(X.nitems), dim3(64), 0, s, X.items, X.leafCell, X.T.pbeg,
-                       X.T.nprt, X.tx, X.ty, X.tz, X.sx, X.sy, X.sz, (const R2*)q, X.rb, X.re,
-                       X.kappa, X.partial);
+    // the near field was launched on the side stream right after the gather;
+    // join it, then fold its partial sums into p
+    HIPCK(hipStreamWaitEvent(s, X.evJoin, 0));
     hipLaunchKernelGGL(k_p2p_reduce<R>, dim3(X.nleaves), dim3(64), 0, s, X.itemBeg, X.items,
                        X.leafCell, X.T.pbeg, X.T.nprt, (const double2*)X.partial, p);
   }
@@ -1093,6 +1139,7 @@ static int runTyped(Exec& X, const double2* qin, double2* pout, int parts, hipSt
                      pout, (int)X.nt);
   mark(9);
   X.evRecorded = true;
+  X.p2pRecorded = near && X.nitems;
   HIPCK(hipGetLastError());
   return OK;
 }
@@ -1175,8 +1222,13 @@ int defmm_plan_create(const double* src_xyz, int64_t n_src, const double* trg_xy
   delete pl;
   if (rc == dfm::OK) {
     if (hipStreamCreate(&h->x.own) != hipSuccess) rc = DEFMM_EHIP;
+    if (rc == dfm::OK && hipStreamCreate(&h->x.side) != hipSuccess) rc = DEFMM_EHIP;
     for (auto& e : h->x.ev)
       if (rc == dfm::OK && hipEventCreate(&e) != hipSuccess) rc = DEFMM_EHIP;
+    for (auto* e : {&h->x.evP2P0, &h->x.evP2P1})
+      if (rc == dfm::OK && hipEventCreate(e) != hipSuccess) rc = DEFMM_EHIP;
+    for (auto* e : {&h->x.evFork, &h->x.evJoin})
+      if (rc == dfm::OK && hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) rc = DEFMM_EHIP;
   }
   if (rc != dfm::OK) {
     delete h;
@@ -1236,10 +1288,31 @@ int defmm_stage_times(defmm_plan plan, double* ms, int n) {
   if (hipEventSynchronize(X.ev[DEFMM_NSTAGES]) != hipSuccess) { g_err = "event sync failed"; return DEFMM_EHIP; }
   for (int k = 0; k < DEFMM_NSTAGES && k < n; k++) {
     float t = 0.f;
-    if (hipEventElapsedTime(&t, X.ev[k], X.ev[k + 1]) != hipSuccess) { g_err = "event timing failed"; return DEFMM_EHIP; }
+    hipEvent_t a = X.ev[k], b = X.ev[k + 1];
+    if (k == 7) {
+      // the near field ran on the side stream, overlapping stages 1..6
+      if (!X.p2pRecorded) { ms[k] = 0.; continue; }
+      a = X.evP2P0;
+      b = X.evP2P1;
+    }
+    if (hipEventElapsedTime(&t, a, b) != hipSuccess) { g_err = "event timing failed"; return DEFMM_EHIP; }
     ms[k] = t;
   }
   return DEFMM_OK;
+}
+
+int defmm_plan_set_precision(defmm_plan plan, int precision) {
+  if (!plan) { g_err = "null plan"; return DEFMM_EARG; }
+  if (precision != DEFMM_FP64 && precision != DEFMM_MIXED) { g_err = "bad precision"; return DEFMM_EARG; }
+  Exec& X = plan->x;
+  if (precision == X.prec) return DEFMM_OK;
+  if (hipSetDevice(X.device) != hipSuccess || hipStreamSynchronize(X.own) != hipSuccess) {
+    g_err = "device synchronisation failed";
+    return DEFMM_EHIP;
+  }
+  X.prec = precision;
+  X.evRecorded = false;
+  return dfm::uploadPrecision(X);
 }
 
 int defmm_plan_device_bytes(defmm_plan plan, int64_t* bytes) {
@@ -1309,6 +1382,7 @@ int defmm_plan_destroy(defmm_plan plan) {
   if (!plan) return DEFMM_OK;
   hipSetDevice(plan->x.device);
   hipStreamSynchronize(plan->x.own);
+  if (plan->x.side) hipStreamSynchronize(plan->x.side);
   delete plan;
   return DEFMM_OK;
 }

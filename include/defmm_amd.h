@@ -109,6 +109,13 @@ int defmm_plan_build_stats(const double* src_xyz, int64_t n_src, const double* t
 int defmm_direct(const double* src_xyz, int64_t n_src, const double* q, const double* trg_xyz,
                  int64_t n_trg, double kappa, double* p);
 
+/*
+ * Switch the arithmetic of an existing plan (DEFMM_FP64 / DEFMM_MIXED): the
+ * lists stay, the expansion stores and tables are reallocated in the new
+ * precision.  Synchronises the plan's work first.
+ */
+int defmm_plan_set_precision(defmm_plan plan, int precision);
+
 /* Bytes of device memory held by the plan. */
 int defmm_plan_device_bytes(defmm_plan plan, int64_t* bytes);
 

@@ -165,3 +165,28 @@ def test_error_codes():
         A.prcmpt(4, 64, 1.0)   # a single cell: no precomputation level
     with pytest.raises(dm.DefmmError, match="invalid argument"):
         A.prcmpt(9, 8, 1.0)    # order above 8
+
+
+def test_cpp_dropin_header_reference_scenario(tmp_path):
+    """A C++ program written against include/interface.hpp (the reference's
+    API) reproduces the reference's printed known answer at L=6."""
+    import gzip
+    import os
+    import re
+    import shutil
+    import subprocess
+    _say("case: C++ drop-in header, refined cube L=6 Ncrit=32")
+    mesh = tmp_path / "refinedcube.inp"
+    with gzip.open(os.path.join(oa.GOLDEN, "refinedcube.inp.gz"), "rb") as f, open(mesh, "wb") as g:
+        shutil.copyfileobj(f, g)
+    exe = tmp_path / "refined_cube"
+    libdir = os.path.join(oa.ROOT, "defmm_amd")
+    subprocess.check_call(["g++", "-O2", "-std=c++17", "-I" + os.path.join(oa.ROOT, "include"),
+                           os.path.join(oa.ROOT, "tests", "cpp", "refined_cube.cpp"), "-L" + libdir,
+                           "-ldefmm_amd", "-Wl,-rpath," + libdir, "-o", str(exe)])
+    out = subprocess.run([str(exe), str(mesh), "6", "32"], capture_output=True, text=True,
+                         timeout=240).stdout
+    out = re.sub(r"\x1b\[[0-9;]*m", "", out)
+    vals = [float(v) for v in re.findall(r"norm: ([0-9.eE+-]+)", out)]
+    _say("  " + " ".join(f"{v:.5e}" for v in vals))
+    np.testing.assert_allclose(vals, [1.38628e-05, 1.22574e-05, 1.13715e-05], rtol=2e-4)
