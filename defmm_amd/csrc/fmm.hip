@@ -20,6 +20,18 @@ namespace dfm {
 
 static thread_local std::string g_err;
 
+// Work-item grids: the dispatch packet counts work-items in 32 bits, so a
+// one-dimensional grid of n blocks x b threads must keep n*b < 2^32.  Lists of
+// millions of items use a 2D grid of at most kGridX blocks per row; a block's
+// item index is blockIdx.y * gridDim.x + blockIdx.x.
+constexpr unsigned kGridX = 32768;
+__device__ __forceinline__ int itemIndex() { return (int)(blockIdx.y * gridDim.x + blockIdx.x); }
+static inline dim3 itemGrid(int64_t n) {
+  unsigned gx = (unsigned)std::min<int64_t>(n, kGridX);
+  unsigned gy = (unsigned)((n + gx - 1) / gx);
+  return dim3(gx, gy);
+}
+
 // ----------------------------------------------------------------------------
 // kernels
 // ----------------------------------------------------------------------------
@@ -76,14 +88,16 @@ __global__ __launch_bounds__(128) void k_p2m(const int* __restrict__ leaves, Dev
                                              const double* __restrict__ dirs,
                                              const typename C2T<R>::T* __restrict__ sig2,
                                              OpConsts<R> oc, double kappa,
-                                             typename C2T<R>::T* __restrict__ M) {
+                                             typename C2T<R>::T* __restrict__ M, int nItems) {
   typedef typename C2T<R>::T R2;
   constexpr int LL = L * L * L;
   constexpr int NI = (LL + 127) / 128;
   __shared__ R Sw[3][L][64];
   __shared__ R2 w[64];
   __shared__ double rel[3][64];
-  const int c = leaves[blockIdx.x];
+  const int item = itemIndex();
+  if (item >= nItems) return;
+  const int c = leaves[item];
   const int lvl = S.lvl[c], pb = S.pbeg[c], n = S.nprt[c], s0 = S.slot0[c], ns = S.nslot[c];
   const double cx = S.ctr[3 * c], cy = S.ctr[3 * c + 1], cz = S.ctr[3 * c + 2], rad = S.rad[c];
   const int hf = lv.isHF[lvl];
@@ -172,7 +186,7 @@ __global__ __launch_bounds__(128) void k_trans(
     const int* __restrict__ sgn, const int* __restrict__ sigIdx, const double2* __restrict__ edv,
     const typename C2T<R>::T* __restrict__ src, typename C2T<R>::T* __restrict__ dst,
     const typename C2T<R>::T* __restrict__ preTab, const typename C2T<R>::T* __restrict__ postTab,
-    const R* __restrict__ mat0, const R* __restrict__ mat1, int accumulate) {
+    const R* __restrict__ mat0, const R* __restrict__ mat1, int accumulate, int nItems) {
   typedef typename C2T<R>::T R2;
   constexpr int LL = L * L * L;
   constexpr int NI = (LL + 127) / 128;
@@ -183,7 +197,8 @@ __global__ __launch_bounds__(128) void k_trans(
     mt[0][x] = mat0[x];
     mt[1][x] = mat1[x];
   }
-  const int o = blockIdx.x;
+  const int o = itemIndex();
+  if (o >= nItems) return;
   const int e0 = beg[o], e1 = beg[o + 1];
   R2 acc[NI];
 #pragma unroll
@@ -285,17 +300,19 @@ __global__ __launch_bounds__(256) void k_m2f(const int* __restrict__ m2fSlot,
                                              const typename C2T<R>::T* __restrict__ M,
                                              typename C2T<R>::T* __restrict__ F,
                                              const R* __restrict__ twc,
-                                             const R* __restrict__ tws) {
+                                             const R* __restrict__ tws, int nItems) {
   typedef typename C2T<R>::T R2;
   constexpr int P = 2 * L - 1, LL = L * L * L, PP = P * P * P;
   __shared__ R2 sM[LL], T1[L * L * P], T2[L * P * P];
   __shared__ R wc[P], ws[P];
   const int tid = threadIdx.x;
+  const int item = itemIndex();
+  if (item >= nItems) return;
   if (tid < P) {
     wc[tid] = twc[tid];
     ws[tid] = tws[tid];
   }
-  const int s = m2fSlot[blockIdx.x];
+  const int s = m2fSlot[item];
   for (int I = tid; I < LL; I += 256) sM[I] = M[(size_t)s * LL + I];
   __syncthreads();
   for (int x = tid; x < L * L * P; x += 256) {
@@ -324,7 +341,7 @@ __global__ __launch_bounds__(256) void k_m2f(const int* __restrict__ m2fSlot,
     T2[x] = t;
   }
   __syncthreads();
-  R2* Fr = F + (size_t)blockIdx.x * PP;
+  R2* Fr = F + (size_t)item * PP;
   for (int x = tid; x < PP; x += 256) {
     const int k0 = x / (P * P), r = x % (P * P);
     R2 t = c2<R>(0, 0);
@@ -353,14 +370,15 @@ __global__ __launch_bounds__(256) void k_m2l(const int* __restrict__ beg,
                                              const int* __restrict__ perm,
                                              typename C2T<R>::T* __restrict__ Lout,
                                              const R* __restrict__ twc,
-                                             const R* __restrict__ tws) {
+                                             const R* __restrict__ tws, int nItems) {
   typedef typename C2T<R>::T R2;
   constexpr int P = 2 * L - 1, LL = L * L * L, PP = P * P * P;
   constexpr int NW = (PP + 255) / 256;
   __shared__ R2 J[PP], T1[L * P * P], T2[L * L * P];
   __shared__ R wc[P], ws[P];
   const int tid = threadIdx.x;
-  const int t = blockIdx.x;
+  const int t = itemIndex();
+  if (t >= nItems) return;
   const int e0 = beg[t], e1 = beg[t + 1];
   R2* Lr = Lout + (size_t)t * LL;
   if (e0 == e1) {
@@ -452,11 +470,13 @@ __global__ __launch_bounds__(64) void k_l2p(const int* __restrict__ leaves, DevC
                                             const typename C2T<R>::T* __restrict__ sig2,
                                             const typename C2T<R>::T* __restrict__ Lst,
                                             OpConsts<R> oc, double kappa,
-                                            typename C2T<R>::T* __restrict__ p) {
+                                            typename C2T<R>::T* __restrict__ p, int nItems) {
   typedef typename C2T<R>::T R2;
   constexpr int LL = L * L * L;
   __shared__ R2 buf[LL];
-  const int c = leaves[blockIdx.x];
+  const int item = itemIndex();
+  if (item >= nItems) return;
+  const int c = leaves[item];
   const int lvl = T.lvl[c], pb = T.pbeg[c], n = T.nprt[c], s0 = T.slot0[c], ns = T.nslot[c];
   const double cx = T.ctr[3 * c], cy = T.ctr[3 * c + 1], cz = T.ctr[3 * c + 2], rad = T.rad[c];
   const int hf = lv.isHF[lvl];
@@ -586,12 +606,15 @@ __global__ __launch_bounds__(64) void k_p2p(const P2PItem* __restrict__ items,
                                             const double* __restrict__ sz,
                                             const typename C2T<R>::T* __restrict__ q,
                                             const int* __restrict__ rb, const int* __restrict__ re,
-                                            double kappa, double2* __restrict__ partial) {
+                                            double kappa, double2* __restrict__ partial,
+                                            int nItems) {
   typedef typename C2T<R>::T R2;
   __shared__ double lx[64], ly[64], lz[64];
   __shared__ R2 lq[64];
   __shared__ double2 red[64];
-  const P2PItem it = items[blockIdx.x];
+  const int item = itemIndex();
+  if (item >= nItems) return;
+  const P2PItem it = items[item];
   const int c = leafCell[it.leaf];
   const int tb = tpbeg[c] + it.t0;
   const int nt = min(64, tnprt[c] - it.t0);
@@ -649,7 +672,7 @@ __global__ __launch_bounds__(64) void k_p2p(const P2PItem* __restrict__ items,
       s.x += red[gg * nt + lane].x;
       s.y += red[gg * nt + lane].y;
     }
-    partial[(size_t)blockIdx.x * 64 + lane] = s;
+    partial[(size_t)item * 64 + lane] = s;
   }
 }
 
@@ -662,9 +685,11 @@ __global__ __launch_bounds__(64) void k_p2p_reduce(const int* __restrict__ itemB
                                                    const int* __restrict__ tpbeg,
                                                    const int* __restrict__ tnprt,
                                                    const double2* __restrict__ partial,
-                                                   typename C2T<R>::T* __restrict__ p) {
+                                                   typename C2T<R>::T* __restrict__ p,
+                                                   int nItems) {
   typedef typename C2T<R>::T R2;
-  const int leaf = blockIdx.x;
+  const int leaf = itemIndex();
+  if (leaf >= nItems) return;
   const int c = leafCell[leaf];
   const int pb = tpbeg[c], n = tnprt[c];
   const int ib = itemBeg[leaf], ie = itemBeg[leaf + 1];
@@ -1094,45 +1119,48 @@ static int runTyped(Exec& X, const double2* qin, double2* pout, int parts, hipSt
     HIPCK(hipEventRecord(X.evFork, s));
     HIPCK(hipStreamWaitEvent(X.side, X.evFork, 0));
     HIPCK(hipEventRecord(X.evP2P0, X.side));
-    hipLaunchKernelGGL(k_p2p<R>, dim3(X.nitems), dim3(64), 0, X.side, X.items, X.leafCell,
+    hipLaunchKernelGGL(k_p2p<R>, itemGrid(X.nitems), dim3(64), 0, X.side, X.items, X.leafCell,
                        X.T.pbeg, X.T.nprt, X.tx, X.ty, X.tz, X.sx, X.sy, X.sz, (const R2*)q, X.rb,
-                       X.re, X.kappa, X.partial);
+                       X.re, X.kappa, X.partial, X.nitems);
     HIPCK(hipEventRecord(X.evP2P1, X.side));
     HIPCK(hipEventRecord(X.evJoin, X.side));
   }
   if (far && X.np2m)
-    hipLaunchKernelGGL((k_p2m<R, L>), dim3(X.np2m), dim3(128), 0, s, X.p2m, X.S, X.lv, X.sx, X.sy,
-                       X.sz, (const R2*)q, X.dirs, sig2, oc, X.kappa, ML);
+    hipLaunchKernelGGL((k_p2m<R, L>), itemGrid(X.np2m), dim3(128), 0, s, X.p2m, X.S, X.lv, X.sx,
+                       X.sy, X.sz, (const R2*)q, X.dirs, sig2, oc, X.kappa, ML, X.np2m);
   mark(2);
   if (far)
     for (auto& t : X.m2m)
-      hipLaunchKernelGGL((k_trans<R, L>), dim3(t.n), dim3(128), 0, s, t.out, t.beg, t.in, t.sgn,
-                         t.sigIdx, t.edv, (const R2*)ML, ML, sig, sig2, oc.A[0], oc.A[1], 0);
+      hipLaunchKernelGGL((k_trans<R, L>), itemGrid(t.n), dim3(128), 0, s, t.out, t.beg, t.in,
+                         t.sgn, t.sigIdx, t.edv, (const R2*)ML, ML, sig, sig2, oc.A[0], oc.A[1], 0,
+                         t.n);
   mark(3);
   if (far && X.nm2f)
-    hipLaunchKernelGGL((k_m2f<R, L>), dim3(X.nm2f), dim3(256), 0, s, X.m2fSlot, (const R2*)ML, F,
-                       oc.twc, oc.tws);
+    hipLaunchKernelGGL((k_m2f<R, L>), itemGrid(X.nm2f), dim3(256), 0, s, X.m2fSlot, (const R2*)ML,
+                       F, oc.twc, oc.tws, X.nm2f);
   mark(4);
   // L (aliasing M) is written for every target slot here
   if (far && X.nTslots)
-    hipLaunchKernelGGL((k_m2l<R, L>), dim3(X.nTslots), dim3(256), 0, s, X.m2lBeg, X.m2lSrcF,
-                       X.m2lTabHash, (const R2*)F, (const R2*)X.U, X.perm, ML, oc.twc, oc.tws);
+    hipLaunchKernelGGL((k_m2l<R, L>), itemGrid(X.nTslots), dim3(256), 0, s, X.m2lBeg, X.m2lSrcF,
+                       X.m2lTabHash, (const R2*)F, (const R2*)X.U, X.perm, ML, oc.twc, oc.tws,
+                       X.nTslots);
   mark(5);
   if (far)
     for (auto& t : X.l2l)
-      hipLaunchKernelGGL((k_trans<R, L>), dim3(t.n), dim3(128), 0, s, t.out, t.beg, t.in, t.sgn,
-                         t.sigIdx, t.edv, (const R2*)ML, ML, sig2, sig, oc.B[0], oc.B[1], 1);
+      hipLaunchKernelGGL((k_trans<R, L>), itemGrid(t.n), dim3(128), 0, s, t.out, t.beg, t.in,
+                         t.sgn, t.sigIdx, t.edv, (const R2*)ML, ML, sig2, sig, oc.B[0], oc.B[1], 1,
+                         t.n);
   mark(6);
   if (far && X.nl2p)
-    hipLaunchKernelGGL((k_l2p<R, L>), dim3(X.nl2p), dim3(64), 0, s, X.l2p, X.T, X.lv, X.tx, X.ty,
-                       X.tz, X.dirs, sig2, (const R2*)ML, oc, X.kappa, p);
+    hipLaunchKernelGGL((k_l2p<R, L>), itemGrid(X.nl2p), dim3(64), 0, s, X.l2p, X.T, X.lv, X.tx,
+                       X.ty, X.tz, X.dirs, sig2, (const R2*)ML, oc, X.kappa, p, X.nl2p);
   mark(7);
   if (near && X.nitems) {
     // the near field was launched on the side stream right after the gather;
     // join it, then fold its partial sums into p
     HIPCK(hipStreamWaitEvent(s, X.evJoin, 0));
-    hipLaunchKernelGGL(k_p2p_reduce<R>, dim3(X.nleaves), dim3(64), 0, s, X.itemBeg, X.items,
-                       X.leafCell, X.T.pbeg, X.T.nprt, (const double2*)X.partial, p);
+    hipLaunchKernelGGL(k_p2p_reduce<R>, itemGrid(X.nleaves), dim3(64), 0, s, X.itemBeg, X.items,
+                       X.leafCell, X.T.pbeg, X.T.nprt, (const double2*)X.partial, p, X.nleaves);
   }
   mark(8);
   hipLaunchKernelGGL(k_scatter<R>, dim3(nblk(X.nt, 256)), dim3(256), 0, s, (const R2*)p, X.tperm,
