@@ -72,6 +72,11 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--samples", type=int, default=2000)
+    ap.add_argument("--both-m2l", action="store_true",
+                    help="also time the M2L variant that recomputes Fourier rows on the fly")
+    ap.add_argument("--copy-gib", type=float, default=0.0,
+                    help="after timing, copy this many GiB device-to-device (a kernel of known "
+                         "traffic, to scale rocprof byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path "
@@ -126,6 +131,12 @@ def main():
         return ms, err2, errinf, avg
 
     ms64, e64, einf64, st64 = run("fp64")
+    alt = None
+    if args.both_m2l:
+        A.set_m2l(False)
+        msa, ea, einfa, sta = run("fp64 m2l-on-the-fly")
+        alt = {"ms_per_step": msa, "rel_l2_err": ea, "stage_ms": sta}
+        A.set_m2l(True)
     value = n / (ms64 * 1e-3) / 1e6
 
     # dominant kernel of the FP64 pass and its roofline
@@ -156,6 +167,15 @@ def main():
                  "rel_linf_err": einfmx, "speedup_vs_fp64": ms64 / msmx, "stage_ms": stmx,
                  "dtype": "f32 storage, f64 phase reduction"}
     A.destroy()
+    if args.copy_gib > 0:
+        nb = int(args.copy_gib * (1 << 30)) // 16
+        src = torch.ones(nb, dtype=torch.complex128, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copied {nb * 16} bytes device-to-device")
+        del src, dst
 
     cpu = None
     if not args.no_cpu:
@@ -179,6 +199,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "mixed": mixed,
+        "fp64_m2l_on_the_fly": alt,
         "plan": {k: st[k] for k in ("p2p_pairs", "hf_m2l", "src_slots", "trg_slots",
                                       "m2f_slots", "m2l_entries", "p2p_items")},
     }

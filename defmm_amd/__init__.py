@@ -73,6 +73,7 @@ def lib():
                                          ctypes.c_int, ctypes.c_double, ctypes.c_int, I64,
                                          ctypes.c_int]
     L.defmm_plan_set_precision.argtypes = [P, ctypes.c_int]
+    L.defmm_plan_set_m2l.argtypes = [P, ctypes.c_int]
     L.defmm_direct.argtypes = [D, ctypes.c_int64, D, D, ctypes.c_int64, ctypes.c_double, D]
     _lib = L
     return L
@@ -207,6 +208,11 @@ class IBFMM_Mat:
         self._need_plan()
         _check(lib().defmm_plan_set_precision(self._plan, int(precision)))
         self.precision = precision
+
+    def set_m2l(self, store_fourier):
+        """M2L from stored Fourier rows (True) or rows computed on the fly (False)."""
+        self._need_plan()
+        _check(lib().defmm_plan_set_m2l(self._plan, int(bool(store_fourier))))
 
     def stats(self):
         self._need_plan()

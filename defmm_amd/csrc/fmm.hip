@@ -356,106 +356,337 @@ __global__ __launch_bounds__(256) void k_m2f(const int* __restrict__ m2fSlot,
   }
 }
 
-// Fused M2L + F2L for one target slot (M2L.hpp:114-157, F2L.hpp:35-46,
-// circulant.hpp:108-118):
-//   J[w] = sum_e U_{tab_e}[perm_{h_e}[w]] * F_{src_e}[w]
-//   L[I] = (1/PP) * IDFT(J)[I2i(I, P)]     (written, not accumulated)
+// Fused M2L + F2L (M2L.hpp:114-157, F2L.hpp:35-46, circulant.hpp:108-118) for
+// a group of up to G target slots of one level and direction:
+//   J_g[w] = sum over the group's entries e read by target g of
+//            U_{tab}[perm_h(w)] * F_{src_e}[w]
+//   L_g[I] = (1/PP) * IDFT(J_g)[I2i(I, P)]          (written, not accumulated)
+// Each entry's F row is loaded once for all targets of the group that read it.
 // J never leaves LDS; the inverse transform only produces the L^3 corner.
+// perm_h (permutations.hpp:127-150) is evaluated arithmetically: with
+// h = z + 8 s, component k of the permuted index is w[sigma_s[k]], replaced
+// by P - w[sigma_s[k]] when bit sigma_s[k] of z is set and it is nonzero.
+__constant__ int c_sigma[8][3] = {{0, 1, 2}, {1, 0, 2}, {0, 1, 2}, {1, 2, 0},
+                                  {0, 2, 1}, {0, 1, 2}, {2, 0, 1}, {2, 1, 0}};
+
+template <int P>
+__device__ __forceinline__ int permIndex(int h, int w0, int w1, int w2) {
+  const int z = h & 7, sg = h >> 3;
+  int a[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const int sk = c_sigma[sg][k];
+    int v = sk == 0 ? w0 : (sk == 1 ? w1 : w2);
+    if (((z >> sk) & 1) && v) v = P - v;
+    a[k] = v;
+  }
+  return (a[0] * P + a[1]) * P + a[2];
+}
+
 template <typename R, int L>
-__global__ __launch_bounds__(256) void k_m2l(const int* __restrict__ beg,
-                                             const int* __restrict__ srcF,
-                                             const int* __restrict__ tabHash,
+__global__ __launch_bounds__(256) void k_m2l(const int* __restrict__ groupBeg,
+                                             const int* __restrict__ groupTarget,
+                                             const int* __restrict__ gSrcF,
+                                             const int* __restrict__ gMask,
+                                             const int* __restrict__ gCode,
                                              const typename C2T<R>::T* __restrict__ F,
                                              const typename C2T<R>::T* __restrict__ U,
-                                             const int* __restrict__ perm,
                                              typename C2T<R>::T* __restrict__ Lout,
                                              const R* __restrict__ twc,
                                              const R* __restrict__ tws, int nItems) {
   typedef typename C2T<R>::T R2;
   constexpr int P = 2 * L - 1, LL = L * L * L, PP = P * P * P;
   constexpr int NW = (PP + 255) / 256;
+  constexpr int G = kM2LGroup;
   __shared__ R2 J[PP], T1[L * P * P], T2[L * L * P];
   __shared__ R wc[P], ws[P];
   const int tid = threadIdx.x;
-  const int t = itemIndex();
-  if (t >= nItems) return;
-  const int e0 = beg[t], e1 = beg[t + 1];
-  R2* Lr = Lout + (size_t)t * LL;
-  if (e0 == e1) {
-    for (int I = tid; I < LL; I += 256) Lr[I] = c2<R>(0, 0);
-    return;
-  }
+  // Workgroups are dealt to the 8 XCDs round-robin; give each XCD a
+  // contiguous part of the (direction-major) group order so that its L2 sees
+  // the groups that share source rows.  A bijection of [0, n8), n8 = n
+  // rounded down to a multiple of 8, and the identity on the tail.
+  const int b = itemIndex();
+  if (b >= nItems) return;
+  const int n8 = nItems & ~7;
+  const int grp = b < n8 ? (b & 7) * (n8 >> 3) + (b >> 3) : b;
   if (tid < P) {
     wc[tid] = twc[tid];
     ws[tid] = tws[tid];
   }
-  R2 acc[NW];
+  int w0[NW], w1[NW], w2[NW];
 #pragma unroll
-  for (int m = 0; m < NW; m++) acc[m] = c2<R>(0, 0);
+  for (int m = 0; m < NW; m++) {
+    const int w = min(tid + 256 * m, PP - 1);
+    w0[m] = w / (P * P);
+    w1[m] = (w / P) % P;
+    w2[m] = w % P;
+  }
+  R2 acc[G][NW];
+#pragma unroll
+  for (int g = 0; g < G; g++)
+#pragma unroll
+    for (int m = 0; m < NW; m++) acc[g][m] = c2<R>(0, 0);
+  const int e0 = groupBeg[grp], e1 = groupBeg[grp + 1];
   for (int e = e0; e < e1; e++) {
-    const int th = tabHash[e];
-    const R2* Fr = F + (size_t)srcF[e] * PP;
-    const R2* Ur = U + (size_t)(th >> 6) * PP;
-    const int* pr = perm + (th & 63) * PP;
+    const R2* Fr = F + (size_t)gSrcF[e] * PP;
+    const int mask = gMask[e];
+    R2 f[NW];
 #pragma unroll
     for (int m = 0; m < NW; m++) {
       const int w = tid + 256 * m;
-      if (w < PP) {
-        const R2 u = Ur[pr[w]];
-        const R2 f = Fr[w];
-        acc[m].x += u.x * f.x - u.y * f.y;
-        acc[m].y += u.x * f.y + u.y * f.x;
+      f[m] = w < PP ? Fr[w] : c2<R>(0, 0);
+    }
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      if (!((mask >> g) & 1)) continue;
+      const int code = gCode[(size_t)e * G + g];
+      const R2* Ur = U + (size_t)(code >> 6) * PP;
+      const int h = code & 63;
+#pragma unroll
+      for (int m = 0; m < NW; m++) {
+        const int w = tid + 256 * m;
+        if (w < PP) {
+          const R2 u = Ur[permIndex<P>(h, w0[m], w1[m], w2[m])];
+          acc[g][m].x += u.x * f[m].x - u.y * f[m].y;
+          acc[g][m].y += u.x * f[m].y + u.y * f[m].x;
+        }
       }
     }
   }
+  const R inv = (R)(1.0 / (double)PP);
+#pragma unroll
+  for (int g = 0; g < G; g++) {
+    const int t = groupTarget[(size_t)grp * G + g];
+    if (t < 0) continue;
+    R2* Lr = Lout + (size_t)t * LL;
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < NW; m++) {
+      const int w = tid + 256 * m;
+      if (w < PP) J[w] = acc[g][m];
+    }
+    __syncthreads();
+    // axis 0: T1[(k0*P + w1)*P + w2], k0 < L
+    for (int x = tid; x < L * P * P; x += 256) {
+      const int r = x % (P * P), k0 = x / (P * P);
+      R2 s = c2<R>(0, 0);
+#pragma unroll
+      for (int v0 = 0; v0 < P; v0++) {
+        const int m = (v0 * k0) % P;
+        const R2 v = J[v0 * P * P + r];
+        s.x += v.x * wc[m] - v.y * ws[m];
+        s.y += v.y * wc[m] + v.x * ws[m];
+      }
+      T1[x] = s;
+    }
+    __syncthreads();
+    // axis 1: T2[(k0*L + k1)*P + w2], k1 < L
+    for (int x = tid; x < L * L * P; x += 256) {
+      const int v2 = x % P, k1 = (x / P) % L, k0 = x / (L * P);
+      R2 s = c2<R>(0, 0);
+#pragma unroll
+      for (int v1 = 0; v1 < P; v1++) {
+        const int m = (v1 * k1) % P;
+        const R2 v = T1[(k0 * P + v1) * P + v2];
+        s.x += v.x * wc[m] - v.y * ws[m];
+        s.y += v.y * wc[m] + v.x * ws[m];
+      }
+      T2[x] = s;
+    }
+    __syncthreads();
+    // axis 2 and scaling: L[(k0*L + k1)*L + k2]
+    for (int x = tid; x < LL; x += 256) {
+      const int k2 = x % L, r = x / L;
+      R2 s = c2<R>(0, 0);
+#pragma unroll
+      for (int v2 = 0; v2 < P; v2++) {
+        const int m = (v2 * k2) % P;
+        const R2 v = T2[r * P + v2];
+        s.x += v.x * wc[m] - v.y * ws[m];
+        s.y += v.y * wc[m] + v.x * ws[m];
+      }
+      Lr[x] = c2<R>(s.x * inv, s.y * inv);
+    }
+  }
+}
+
+// The same grouped M2L + F2L without a stored F: each entry's source row is
+// transformed on the fly from its M expansion (the M2F of M2F.hpp:35-46,
+// circulant.hpp:95-106: pruned 3-pass DFT of the embedded L^3 corner, in
+// LDS), so only M (L^3) is read per entry and no F store exists.
+template <typename R, int L>
+__global__ __launch_bounds__(256) void k_m2l_fly(const int* __restrict__ groupBeg,
+                                                 const int* __restrict__ groupTarget,
+                                                 const int* __restrict__ gSrcSlot,
+                                                 const int* __restrict__ gMask,
+                                                 const int* __restrict__ gCode,
+                                                 const typename C2T<R>::T* __restrict__ M,
+                                                 const typename C2T<R>::T* __restrict__ U,
+                                                 typename C2T<R>::T* __restrict__ Lout,
+                                                 const R* __restrict__ twc,
+                                                 const R* __restrict__ tws, int nItems) {
+  typedef typename C2T<R>::T R2;
+  constexpr int P = 2 * L - 1, LL = L * L * L, PP = P * P * P;
+  constexpr int NW = (PP + 255) / 256;
+  constexpr int G = kM2LGroup;
+  constexpr int NB = L * P * P;  // >= L*L*P
+  __shared__ R2 sM[LL], B1[NB], B2[NB], J[PP];
+  __shared__ R wc[P], ws[P];
+  const int tid = threadIdx.x;
+  const int b = itemIndex();
+  if (b >= nItems) return;
+  const int n8 = nItems & ~7;
+  const int grp = b < n8 ? (b & 7) * (n8 >> 3) + (b >> 3) : b;
+  if (tid < P) {
+    wc[tid] = twc[tid];
+    ws[tid] = tws[tid];
+  }
+  int w0[NW], w1[NW], w2[NW];
 #pragma unroll
   for (int m = 0; m < NW; m++) {
-    const int w = tid + 256 * m;
-    if (w < PP) J[w] = acc[m];
+    const int w = min(tid + 256 * m, PP - 1);
+    w0[m] = w / (P * P);
+    w1[m] = (w / P) % P;
+    w2[m] = w % P;
   }
-  __syncthreads();
-  // axis 0: T1[(k0*P + w1)*P + w2], k0 < L
-  for (int x = tid; x < L * P * P; x += 256) {
-    const int r = x % (P * P), k0 = x / (P * P);
-    R2 s = c2<R>(0, 0);
+  R2 acc[G][NW];
 #pragma unroll
-    for (int w0 = 0; w0 < P; w0++) {
-      const int m = (w0 * k0) % P;
-      const R2 v = J[w0 * P * P + r];
-      s.x += v.x * wc[m] - v.y * ws[m];
-      s.y += v.y * wc[m] + v.x * ws[m];
-    }
-    T1[x] = s;
-  }
-  __syncthreads();
-  // axis 1: T2[(k0*L + k1)*P + w2], k1 < L
-  for (int x = tid; x < L * L * P; x += 256) {
-    const int w2 = x % P, k1 = (x / P) % L, k0 = x / (L * P);
-    R2 s = c2<R>(0, 0);
+  for (int g = 0; g < G; g++)
 #pragma unroll
-    for (int w1 = 0; w1 < P; w1++) {
-      const int m = (w1 * k1) % P;
-      const R2 v = T1[(k0 * P + w1) * P + w2];
-      s.x += v.x * wc[m] - v.y * ws[m];
-      s.y += v.y * wc[m] + v.x * ws[m];
+    for (int m = 0; m < NW; m++) acc[g][m] = c2<R>(0, 0);
+  const int e0 = groupBeg[grp], e1 = groupBeg[grp + 1];
+  for (int e = e0; e < e1; e++) {
+    const R2* Mr = M + (size_t)gSrcSlot[e] * LL;
+    const int mask = gMask[e];
+    __syncthreads();
+    for (int I = tid; I < LL; I += 256) sM[I] = Mr[I];
+    __syncthreads();
+    // axis 2: B1[(i0*L + i1)*P + k2]
+    for (int x = tid; x < L * L * P; x += 256) {
+      const int k2 = x % P, r = x / P;
+      R2 t = c2<R>(0, 0);
+#pragma unroll
+      for (int j = 0; j < L; j++) {
+        const int m = (j * k2) % P;
+        const R2 v = sM[r * L + j];
+        t.x += v.x * wc[m] + v.y * ws[m];
+        t.y += v.y * wc[m] - v.x * ws[m];
+      }
+      B1[x] = t;
     }
-    T2[x] = s;
+    __syncthreads();
+    // axis 1: B2[(i0*P + k1)*P + k2]
+    for (int x = tid; x < L * P * P; x += 256) {
+      const int k2 = x % P, k1 = (x / P) % P, i0 = x / (P * P);
+      R2 t = c2<R>(0, 0);
+#pragma unroll
+      for (int j = 0; j < L; j++) {
+        const int m = (j * k1) % P;
+        const R2 v = B1[(i0 * L + j) * P + k2];
+        t.x += v.x * wc[m] + v.y * ws[m];
+        t.y += v.y * wc[m] - v.x * ws[m];
+      }
+      B2[x] = t;
+    }
+    __syncthreads();
+    // axis 0 for this thread's frequencies, then the Hadamard products
+    R2 f[NW];
+#pragma unroll
+    for (int m = 0; m < NW; m++) {
+      const int w = tid + 256 * m;
+      R2 t = c2<R>(0, 0);
+      if (w < PP) {
+        const int r = w % (P * P);
+#pragma unroll
+        for (int j = 0; j < L; j++) {
+          const int mm = (j * w0[m]) % P;
+          const R2 v = B2[j * P * P + r];
+          t.x += v.x * wc[mm] + v.y * ws[mm];
+          t.y += v.y * wc[mm] - v.x * ws[mm];
+        }
+      }
+      f[m] = t;
+    }
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      if (!((mask >> g) & 1)) continue;
+      const int code = gCode[(size_t)e * G + g];
+      const R2* Ur = U + (size_t)(code >> 6) * PP;
+      const int h = code & 63;
+#pragma unroll
+      for (int m = 0; m < NW; m++) {
+        const int w = tid + 256 * m;
+        if (w < PP) {
+          const R2 u = Ur[permIndex<P>(h, w0[m], w1[m], w2[m])];
+          acc[g][m].x += u.x * f[m].x - u.y * f[m].y;
+          acc[g][m].y += u.x * f[m].y + u.y * f[m].x;
+        }
+      }
+    }
   }
-  __syncthreads();
-  // axis 2 and scaling: L[(k0*L + k1)*L + k2]
   const R inv = (R)(1.0 / (double)PP);
-  for (int x = tid; x < LL; x += 256) {
-    const int k2 = x % L, r = x / L;
-    R2 s = c2<R>(0, 0);
 #pragma unroll
-    for (int w2 = 0; w2 < P; w2++) {
-      const int m = (w2 * k2) % P;
-      const R2 v = T2[r * P + w2];
-      s.x += v.x * wc[m] - v.y * ws[m];
-      s.y += v.y * wc[m] + v.x * ws[m];
+  for (int g = 0; g < G; g++) {
+    const int t = groupTarget[(size_t)grp * G + g];
+    if (t < 0) continue;
+    R2* Lr = Lout + (size_t)t * LL;
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < NW; m++) {
+      const int w = tid + 256 * m;
+      if (w < PP) J[w] = acc[g][m];
     }
-    Lr[x] = c2<R>(s.x * inv, s.y * inv);
+    __syncthreads();
+    for (int x = tid; x < L * P * P; x += 256) {
+      const int r = x % (P * P), k0 = x / (P * P);
+      R2 s = c2<R>(0, 0);
+#pragma unroll
+      for (int v0 = 0; v0 < P; v0++) {
+        const int m = (v0 * k0) % P;
+        const R2 v = J[v0 * P * P + r];
+        s.x += v.x * wc[m] - v.y * ws[m];
+        s.y += v.y * wc[m] + v.x * ws[m];
+      }
+      B1[x] = s;
+    }
+    __syncthreads();
+    for (int x = tid; x < L * L * P; x += 256) {
+      const int v2 = x % P, k1 = (x / P) % L, k0 = x / (L * P);
+      R2 s = c2<R>(0, 0);
+#pragma unroll
+      for (int v1 = 0; v1 < P; v1++) {
+        const int m = (v1 * k1) % P;
+        const R2 v = B1[(k0 * P + v1) * P + v2];
+        s.x += v.x * wc[m] - v.y * ws[m];
+        s.y += v.y * wc[m] + v.x * ws[m];
+      }
+      B2[x] = s;
+    }
+    __syncthreads();
+    for (int x = tid; x < LL; x += 256) {
+      const int k2 = x % L, r = x / L;
+      R2 s = c2<R>(0, 0);
+#pragma unroll
+      for (int v2 = 0; v2 < P; v2++) {
+        const int m = (v2 * k2) % P;
+        const R2 v = B2[r * P + v2];
+        s.x += v.x * wc[m] - v.y * ws[m];
+        s.y += v.y * wc[m] + v.x * ws[m];
+      }
+      Lr[x] = c2<R>(s.x * inv, s.y * inv);
+    }
   }
+}
+
+// L = 0 for the target slots that receive no M2L (their L gets only L2L).
+template <typename R, int L>
+__global__ __launch_bounds__(256) void k_zero_slots(const int* __restrict__ slots, int n,
+                                                    typename C2T<R>::T* __restrict__ Lout) {
+  constexpr int LL = L * L * L;
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (int64_t)n * LL) return;
+  const int s = slots[i / LL];
+  Lout[(size_t)s * LL + (i % LL)] = c2<R>(0, 0);
 }
 
 // L2P (L2P.hpp:121-173 with mapOnTrees.hpp:85-134):
@@ -814,6 +1045,10 @@ struct Exec {
   int *m2fSlot = nullptr;
   int nm2f = 0;
   int *m2lBeg = nullptr, *m2lSrcF = nullptr, *m2lTabHash = nullptr;
+  int *m2lOrder = nullptr, *m2lZero = nullptr;
+  int nM2LTargets = 0, nM2LZero = 0;
+  int *gBeg = nullptr, *gTarget = nullptr, *gSrcF = nullptr, *gMask = nullptr, *gCode = nullptr;
+  int nGroups = 0;
   int nTslots = 0, nSslots = 0;
   P2PItem* items = nullptr;
   int nitems = 0;
@@ -821,6 +1056,9 @@ struct Exec {
   int nleaves = 0;
   double2* partial = nullptr;
   void *q = nullptr, *p = nullptr, *ML = nullptr, *F = nullptr;
+  void* Lb = nullptr;        // local expansions (aliases ML when F is stored)
+  bool storeF = true;        // M2L reads a stored F (else: F from M on the fly)
+  int* gSrcSlot = nullptr;   // grouped M2L entries: source slot (for the M rows)
   double2 *qio = nullptr, *pio = nullptr;
   double stageMs[DEFMM_NSTAGES] = {};
   // host copies needed to re-upload the typed buffers in another precision
@@ -932,8 +1170,18 @@ static int uploadTyped(Exec& X) {
     CKR(X.upv(d, c));
     X.consts = d;
   }
-  CKR(X.alloc(X.ML, X.nMLslots * X.LL * sizeof(R2)));
-  CKR(X.alloc(X.F, (size_t)X.nm2f * X.PP * sizeof(R2)));
+  if (X.storeF) {
+    // M is dead after M2F and every target slot's L is written by M2L or
+    // zeroed: one buffer holds M, then L
+    CKR(X.alloc(X.ML, X.nMLslots * X.LL * sizeof(R2)));
+    CKR(X.alloc(X.F, (size_t)X.nm2f * X.PP * sizeof(R2)));
+    X.Lb = X.ML;
+  } else {
+    // M2L reads M while writing L: two buffers, no F
+    CKR(X.alloc(X.ML, (size_t)std::max(1, X.nSslots) * X.LL * sizeof(R2)));
+    CKR(X.alloc(X.Lb, (size_t)std::max(1, X.nTslots) * X.LL * sizeof(R2)));
+    X.F = nullptr;
+  }
   CKR(X.alloc(X.q, (size_t)X.ns * sizeof(R2)));
   CKR(X.alloc(X.p, (size_t)X.nt * sizeof(R2)));
   X.typedPhase = false;
@@ -1008,6 +1256,18 @@ static int upload(Exec& X, Plan& pl) {
   }
   CKR(X.upv(X.m2fSlot, pl.m2fSlot)); X.nm2f = (int)pl.m2fSlot.size();
   CKR(X.upv(X.m2lBeg, pl.m2lBeg)); CKR(X.upv(X.m2lSrcF, pl.m2lSrcF));
+  CKR(X.upv(X.m2lOrder, pl.m2lOrder)); X.nM2LTargets = (int)pl.m2lOrder.size();
+  CKR(X.upv(X.m2lZero, pl.m2lZero)); X.nM2LZero = (int)pl.m2lZero.size();
+  CKR(X.upv(X.gBeg, pl.m2lGroupBeg)); X.nGroups = (int)pl.m2lGroupBeg.size() - 1;
+  CKR(X.upv(X.gTarget, pl.m2lGroupTarget));
+  CKR(X.upv(X.gSrcF, pl.m2lGSrcF));
+  CKR(X.upv(X.gMask, pl.m2lGMask));
+  CKR(X.upv(X.gCode, pl.m2lGCode));
+  {
+    std::vector<int> ss(pl.m2lGSrcF.size());
+    for (size_t e = 0; e < ss.size(); e++) ss[e] = pl.m2fSlot[pl.m2lGSrcF[e]];
+    CKR(X.upv(X.gSrcSlot, ss));
+  }
   {
     std::vector<int> th(pl.m2lTabId.size());
     for (size_t k = 0; k < th.size(); k++) th[k] = (pl.m2lTabId[k] << 6) | pl.m2lHash[k];
@@ -1104,6 +1364,7 @@ static int runTyped(Exec& X, const double2* qin, double2* pout, int parts, hipSt
   R2* q = (R2*)X.q;
   R2* p = (R2*)X.p;
   R2* ML = (R2*)X.ML;
+  R2* Lb = (R2*)X.Lb;
   R2* F = (R2*)X.F;
   const R2* sig = (const R2*)X.sig;
   const R2* sig2 = (const R2*)X.sig2;
@@ -1135,25 +1396,34 @@ static int runTyped(Exec& X, const double2* qin, double2* pout, int parts, hipSt
                          t.sgn, t.sigIdx, t.edv, (const R2*)ML, ML, sig, sig2, oc.A[0], oc.A[1], 0,
                          t.n);
   mark(3);
-  if (far && X.nm2f)
+  if (far && X.nm2f && X.storeF)
     hipLaunchKernelGGL((k_m2f<R, L>), itemGrid(X.nm2f), dim3(256), 0, s, X.m2fSlot, (const R2*)ML,
                        F, oc.twc, oc.tws, X.nm2f);
   mark(4);
   // L (aliasing M) is written for every target slot here
-  if (far && X.nTslots)
-    hipLaunchKernelGGL((k_m2l<R, L>), itemGrid(X.nTslots), dim3(256), 0, s, X.m2lBeg, X.m2lSrcF,
-                       X.m2lTabHash, (const R2*)F, (const R2*)X.U, X.perm, ML, oc.twc, oc.tws,
-                       X.nTslots);
+  if (far && X.nGroups) {
+    if (X.storeF)
+      hipLaunchKernelGGL((k_m2l<R, L>), itemGrid(X.nGroups), dim3(256), 0, s, X.gBeg, X.gTarget,
+                         X.gSrcF, X.gMask, X.gCode, (const R2*)F, (const R2*)X.U, Lb, oc.twc,
+                         oc.tws, X.nGroups);
+    else
+      hipLaunchKernelGGL((k_m2l_fly<R, L>), itemGrid(X.nGroups), dim3(256), 0, s, X.gBeg,
+                         X.gTarget, X.gSrcSlot, X.gMask, X.gCode, (const R2*)ML,
+                         (const R2*)X.U, Lb, oc.twc, oc.tws, X.nGroups);
+  }
+  if (far && X.nM2LZero)
+    hipLaunchKernelGGL((k_zero_slots<R, L>), dim3(nblk((int64_t)X.nM2LZero * X.LL, 256)),
+                       dim3(256), 0, s, X.m2lZero, X.nM2LZero, Lb);
   mark(5);
   if (far)
     for (auto& t : X.l2l)
       hipLaunchKernelGGL((k_trans<R, L>), itemGrid(t.n), dim3(128), 0, s, t.out, t.beg, t.in,
-                         t.sgn, t.sigIdx, t.edv, (const R2*)ML, ML, sig2, sig, oc.B[0], oc.B[1], 1,
+                         t.sgn, t.sigIdx, t.edv, (const R2*)Lb, Lb, sig2, sig, oc.B[0], oc.B[1], 1,
                          t.n);
   mark(6);
   if (far && X.nl2p)
     hipLaunchKernelGGL((k_l2p<R, L>), itemGrid(X.nl2p), dim3(64), 0, s, X.l2p, X.T, X.lv, X.tx,
-                       X.ty, X.tz, X.dirs, sig2, (const R2*)ML, oc, X.kappa, p, X.nl2p);
+                       X.ty, X.tz, X.dirs, sig2, (const R2*)Lb, oc, X.kappa, p, X.nl2p);
   mark(7);
   if (near && X.nitems) {
     // the near field was launched on the side stream right after the gather;
@@ -1339,6 +1609,21 @@ int defmm_plan_set_precision(defmm_plan plan, int precision) {
     return DEFMM_EHIP;
   }
   X.prec = precision;
+  X.evRecorded = false;
+  return dfm::uploadPrecision(X);
+}
+
+int defmm_plan_set_m2l(defmm_plan plan, int store_fourier) {
+  if (!plan) { g_err = "null plan"; return DEFMM_EARG; }
+  Exec& X = plan->x;
+  bool want = store_fourier != 0;
+  if (want == X.storeF) return DEFMM_OK;
+  if (hipSetDevice(X.device) != hipSuccess || hipStreamSynchronize(X.own) != hipSuccess ||
+      hipStreamSynchronize(X.side) != hipSuccess) {
+    g_err = "device synchronisation failed";
+    return DEFMM_EHIP;
+  }
+  X.storeF = want;
   X.evRecorded = false;
   return dfm::uploadPrecision(X);
 }

@@ -737,6 +737,75 @@ int buildPlan(Plan& pl, const double* src, int64_t ns, const double* trg, int64_
   pl.st.m2l_entries = (int64_t)nm;
   for (int s = 0; s < pl.T.nslots; s++)
     if (pl.m2lBeg[s + 1] > pl.m2lBeg[s]) pl.st.m2l_targets++;
+  // Processing order of the M2L targets: by level, then direction, then
+  // cell (pre-order = Morton order).  Targets of one direction in nearby
+  // cells read largely the same source F rows, so running them together
+  // keeps those rows in cache.  Targets without M2L only need L = 0.
+  {
+    std::vector<int> ord;
+    for (int s = 0; s < pl.T.nslots; s++) {
+      if (pl.m2lBeg[s + 1] > pl.m2lBeg[s]) ord.push_back(s);
+      else pl.m2lZero.push_back(s);
+    }
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
+      int la = pl.T.lvl[pl.T.slotCell[a]], lb = pl.T.lvl[pl.T.slotCell[b]];
+      if (la != lb) return la < lb;
+      return pl.T.slotDir[a] < pl.T.slotDir[b];
+    });
+    pl.m2lOrder.swap(ord);
+  }
+  // Groups of up to kM2LGroup consecutive targets of one level and direction
+  // (neighbouring cells): a group's entries are the union of its targets'
+  // source rows, each with the mask of targets that read it and their
+  // (table, permutation) codes, so that a source row is loaded once per group.
+  {
+    const int G = kM2LGroup;
+    pl.m2lGroupBeg.push_back(0);
+    std::vector<std::array<int, 3>> tmp;  // srcF, target in group, tabHash
+    size_t i = 0;
+    while (i < pl.m2lOrder.size()) {
+      int t0 = pl.m2lOrder[i];
+      int l0 = pl.T.lvl[pl.T.slotCell[t0]], d0 = pl.T.slotDir[t0];
+      int ng = 0;
+      int tg[kM2LGroup];
+      while (ng < G && i < pl.m2lOrder.size()) {
+        int t = pl.m2lOrder[i];
+        if (pl.T.lvl[pl.T.slotCell[t]] != l0 || pl.T.slotDir[t] != d0) break;
+        tg[ng++] = t;
+        i++;
+      }
+      tmp.clear();
+      for (int g = 0; g < ng; g++)
+        for (int e = pl.m2lBeg[tg[g]]; e < pl.m2lBeg[tg[g] + 1]; e++)
+          tmp.push_back({pl.m2lSrcF[e], g, (pl.m2lTabId[e] << 6) | pl.m2lHash[e]});
+      std::stable_sort(tmp.begin(), tmp.end(),
+                       [](const std::array<int, 3>& a, const std::array<int, 3>& b) {
+                         return a[0] < b[0];
+                       });
+      for (int g = 0; g < G; g++) pl.m2lGroupTarget.push_back(g < ng ? tg[g] : -1);
+      size_t k = 0;
+      while (k < tmp.size()) {
+        int src = tmp[k][0];
+        int mask = 0;
+        int codes[kM2LGroup] = {0};
+        size_t k2 = k;
+        // a target reads a source row at most once per group entry; a repeat
+        // (the same pair listed twice) starts a new entry
+        while (k2 < tmp.size() && tmp[k2][0] == src && !((mask >> tmp[k2][1]) & 1)) {
+          mask |= 1 << tmp[k2][1];
+          codes[tmp[k2][1]] = tmp[k2][2];
+          k2++;
+        }
+        pl.m2lGSrcF.push_back(src);
+        pl.m2lGMask.push_back(mask);
+        for (int g = 0; g < G; g++) pl.m2lGCode.push_back(codes[g]);
+        k = k2;
+      }
+      pl.m2lGroupBeg.push_back((int)pl.m2lGSrcF.size());
+    }
+    pl.st.m2l_groups = (int64_t)pl.m2lGroupBeg.size() - 1;
+    pl.st.m2l_group_entries = (int64_t)pl.m2lGSrcF.size();
+  }
 
   // ---- P2P, CSR by target leaf (P2P.hpp:42-79 over whole cell ranges) ----
   // A pair with a non-leaf target cell is split over its descendant leaves;
@@ -863,6 +932,45 @@ int validatePlan(const Plan& pl, std::string& msg) {
   }
   for (int v : pl.perm)
     if (v < 0 || v >= pl.PP) return bad("permutation");
+  if (pl.m2lOrder.size() + pl.m2lZero.size() != (size_t)pl.T.nslots) return bad("m2l order size");
+  for (int v : pl.m2lOrder)
+    if (v < 0 || v >= pl.T.nslots) return bad("m2l order");
+  for (int v : pl.m2lZero)
+    if (v < 0 || v >= pl.T.nslots) return bad("m2l zero list");
+  {
+    size_t ng = pl.m2lGroupBeg.size() - 1;
+    if (pl.m2lGroupTarget.size() != ng * kM2LGroup) return bad("m2l group targets");
+    if (pl.m2lGroupBeg.back() != (int)pl.m2lGSrcF.size() ||
+        pl.m2lGMask.size() != pl.m2lGSrcF.size() ||
+        pl.m2lGCode.size() != pl.m2lGSrcF.size() * kM2LGroup)
+      return bad("m2l group csr");
+    std::vector<char> seen(pl.T.nslots, 0);
+    for (size_t g = 0; g < ng; g++) {
+      if (pl.m2lGroupBeg[g + 1] < pl.m2lGroupBeg[g]) return bad("m2l group order");
+      int have = 0;
+      for (int k = 0; k < kM2LGroup; k++) {
+        int t = pl.m2lGroupTarget[g * kM2LGroup + k];
+        if (t < -1 || t >= pl.T.nslots) return bad("m2l group target");
+        if (t >= 0) {
+          if (seen[t]) return bad("m2l target in two groups");
+          seen[t] = 1;
+          have |= 1 << k;
+        }
+      }
+      for (int e = pl.m2lGroupBeg[g]; e < pl.m2lGroupBeg[g + 1]; e++) {
+        if (pl.m2lGSrcF[e] < 0 || pl.m2lGSrcF[e] >= (int)pl.m2fSlot.size()) return bad("m2l group source");
+        int m = pl.m2lGMask[e];
+        if (m <= 0 || (m & ~have)) return bad("m2l group mask");
+        for (int k = 0; k < kM2LGroup; k++) {
+          if (!((m >> k) & 1)) continue;
+          int c = pl.m2lGCode[(size_t)e * kM2LGroup + k];
+          if ((c >> 6) < 0 || (c >> 6) >= pl.ntab) return bad("m2l group table");
+        }
+      }
+    }
+    for (int v : pl.m2lOrder)
+      if (!seen[v]) return bad("m2l target not in a group");
+  }
   if (pl.p2pBeg.size() != pl.p2pLeaf.size() + 1) return bad("p2p csr");
   for (size_t k = 0; k < pl.p2pLeaf.size(); k++) {
     int c = pl.p2pLeaf[k];

@@ -16,6 +16,8 @@ namespace dfm {
 
 typedef std::complex<double> cplx;
 
+constexpr int kM2LGroup = 4;  // target slots per M2L work group
+
 // Error codes returned through the C ABI.
 enum {
   OK = 0,
@@ -59,6 +61,7 @@ struct Stats {
   int64_t hf_l2l = 0, hf2lf_l2l = 0, lf_l2l = 0, nlvl = 0, cpt = 0, max_stage = 0;
   int64_t ncells_t = 0, ncells_s = 0;
   int64_t p2p_items = 0, m2f_slots = 0, m2l_entries = 0, m2l_targets = 0;
+  int64_t m2l_groups = 0, m2l_group_entries = 0;
 };
 
 // A per-level translation list (M2M or L2L): for every output slot, the
@@ -114,6 +117,12 @@ struct Plan {
   std::vector<int> m2lSrcF;     // F row of the source slot
   std::vector<int> m2lTabId;    // kernel table
   std::vector<int> m2lHash;     // permutation
+  std::vector<int> m2lOrder;    // target slots with M2L entries, in processing order
+  std::vector<int> m2lZero;     // target slots without M2L entries (L = 0)
+  // grouped M2L: per group kM2LGroup target slots (-1 padded); per entry a
+  // source F row, the mask of the group's targets reading it and their codes
+  // (table id << 6 | permutation hash), kM2LGroup per entry
+  std::vector<int> m2lGroupBeg, m2lGroupTarget, m2lGSrcF, m2lGMask, m2lGCode;
   // P2P, CSR by target leaf: source particle ranges [b, e)
   std::vector<int> p2pLeaf;     // target leaf cells that receive near field
   std::vector<int> p2pBeg;      // CSR

@@ -116,6 +116,14 @@ int defmm_direct(const double* src_xyz, int64_t n_src, const double* q, const do
  */
 int defmm_plan_set_precision(defmm_plan plan, int precision);
 
+/*
+ * M2L variant: store_fourier = 1 keeps the Fourier rows F of every source
+ * slot read by M2L in HBM (M2F stage + streamed F); 0 recomputes each row
+ * from its M expansion inside M2L (no F store: (2L-1)^3 / L^3 less memory).
+ * Results agree to roundoff.  Reallocates the expansion stores.
+ */
+int defmm_plan_set_m2l(defmm_plan plan, int store_fourier);
+
 /* Bytes of device memory held by the plan. */
 int defmm_plan_device_bytes(defmm_plan plan, int64_t* bytes);
 

@@ -35,12 +35,14 @@ def _diff(a, b):
     return np.sqrt((d ** 2).sum() / (np.abs(b) ** 2).sum()), d.max() / np.abs(b).max()
 
 
-def _run_gpu(src, trg, q, L, ncrit, kappa, prec=0, parts=3):
+def _run_gpu(src, trg, q, L, ncrit, kappa, prec=0, parts=3, store_fourier=None):
     dm = _gpu()
     A = dm.IBFMM_Mat(precision=prec)
     A.setSourceParticles(src)
     A.setTargetParticles(trg)
     A.prcmpt(L, ncrit, kappa)
+    if store_fourier is not None:
+        A.set_m2l(store_fourier)
     p = A.apply(q, parts=parts)
     st = A.stats()
     A.destroy()
@@ -135,6 +137,26 @@ def test_mixed_precision_close_to_fp64(cube, cube_charges, cube_exact):
     _say(f"  mixed vs fp64 rel L2 {l2:.3e} max {mx:.3e}; vs direct L2 {e2:.4e}")
     assert l2 <= 2e-6
     assert abs(e2 - 1.13715e-05) / 1.13715e-05 < 0.05
+
+
+@pytest.mark.parametrize("store", [True, False])
+def test_m2l_variants_match_oracle(cube, cube_charges, store):
+    """M2L from stored Fourier rows and from rows recomputed on the fly."""
+    _say(f"case: refined cube L=4 fp64, M2L store_fourier={store}")
+    po, _, _, _ = oa.fmm(cube, cube, cube_charges, 4, 32, 1.0, mode=2)
+    pg, _ = _run_gpu(cube, cube, cube_charges, 4, 32, 1.0, parts=2, store_fourier=store)
+    l2, mx = _diff(pg, po)
+    _say(f"  far field rel L2 {l2:.3e} rel max {mx:.3e}")
+    assert l2 <= FP64_L2 and mx <= FP64_MAX
+    from defmm_amd import workloads as wl
+    src = wl.sphere(3000, seed=9)
+    q = wl.charges(3000, seed=2)
+    for L in (3, 7):
+        po, _, _, _ = oa.fmm(src, src, q, L, 16, 30.0, mode=2)
+        pg, _ = _run_gpu(src, src, q, L, 16, 30.0, parts=2, store_fourier=store)
+        l2, mx = _diff(pg, po)
+        _say(f"  sphere N=3000 L={L} kappa=30 far field rel L2 {l2:.3e} rel max {mx:.3e}")
+        assert l2 <= FP64_L2 and mx <= FP64_MAX
 
 
 def test_repeated_execution_is_identical(cube, cube_charges):
