@@ -366,25 +366,40 @@ __global__ __launch_bounds__(256) void k_m2f(const int* __restrict__ m2fSlot,
 // perm_h (permutations.hpp:127-150) is evaluated arithmetically: with
 // h = z + 8 s, component k of the permuted index is w[sigma_s[k]], replaced
 // by P - w[sigma_s[k]] when bit sigma_s[k] of z is set and it is nonzero.
-__constant__ int c_sigma[8][3] = {{0, 1, 2}, {1, 0, 2}, {0, 1, 2}, {1, 2, 0},
-                                  {0, 2, 1}, {0, 1, 2}, {2, 0, 1}, {2, 1, 0}};
+// The kernel tables are stored in the 6 axis orders of the symmetry group:
+// variant v of table t holds V[u] = U_t[(u_{s0}, u_{s1}, u_{s2})] for the axis
+// permutation s = sigma(v) (permutations.hpp:87-120).  Then
+// U_t[perm_h(w)] = V[flip_z(w)] with h = z + 8 s (permutations.hpp:127-150):
+// component j of w is replaced by P - w_j when bit j of z is set and w_j != 0.
+// Only sign flips remain, which keep a row of w contiguous (reversed), so the
+// gather reads whole cache lines.
+__constant__ int c_variant[8] = {0, 1, 0, 2, 3, 0, 4, 5};
+
+// M2L block shape for order L: 256 threads, NW frequencies per thread, G
+// target slots per group (measured faster at sphere 1M / 128 wavelengths than
+// 768-thread blocks with one frequency per thread, or groups of 8)
+template <int L>
+struct M2LShape {
+  static constexpr int PP = (2 * L - 1) * (2 * L - 1) * (2 * L - 1);
+  static constexpr int TB = 256;
+  static constexpr int NW = (PP + TB - 1) / TB;
+  static constexpr int G = kM2LGroupMax / 2;
+};
 
 template <int P>
-__device__ __forceinline__ int permIndex(int h, int w0, int w1, int w2) {
-  const int z = h & 7, sg = h >> 3;
-  int a[3];
-#pragma unroll
-  for (int k = 0; k < 3; k++) {
-    const int sk = c_sigma[sg][k];
-    int v = sk == 0 ? w0 : (sk == 1 ? w1 : w2);
-    if (((z >> sk) & 1) && v) v = P - v;
-    a[k] = v;
-  }
-  return (a[0] * P + a[1]) * P + a[2];
+__device__ __forceinline__ int flipIndex(int z, int w0, int w1, int w2) {
+  const int a0 = ((z & 1) && w0) ? P - w0 : w0;
+  const int a1 = ((z & 2) && w1) ? P - w1 : w1;
+  const int a2 = ((z & 4) && w2) ? P - w2 : w2;
+  return (a0 * P + a1) * P + a2;
+}
+template <int PP>
+__device__ __forceinline__ size_t tableRow(int code) {
+  return ((size_t)(code >> 6) * 6 + (size_t)c_variant[(code >> 3) & 7]) * PP;
 }
 
 template <typename R, int L>
-__global__ __launch_bounds__(256) void k_m2l(const int* __restrict__ groupBeg,
+__global__ __launch_bounds__(M2LShape<L>::TB) void k_m2l(const int* __restrict__ groupBeg,
                                              const int* __restrict__ groupTarget,
                                              const int* __restrict__ gSrcF,
                                              const int* __restrict__ gMask,
@@ -396,8 +411,7 @@ __global__ __launch_bounds__(256) void k_m2l(const int* __restrict__ groupBeg,
                                              const R* __restrict__ tws, int nItems) {
   typedef typename C2T<R>::T R2;
   constexpr int P = 2 * L - 1, LL = L * L * L, PP = P * P * P;
-  constexpr int NW = (PP + 255) / 256;
-  constexpr int G = kM2LGroup;
+  constexpr int NW = M2LShape<L>::NW, TB = M2LShape<L>::TB, G = M2LShape<L>::G;
   __shared__ R2 J[PP], T1[L * P * P], T2[L * L * P];
   __shared__ R wc[P], ws[P];
   const int tid = threadIdx.x;
@@ -416,7 +430,7 @@ __global__ __launch_bounds__(256) void k_m2l(const int* __restrict__ groupBeg,
   int w0[NW], w1[NW], w2[NW];
 #pragma unroll
   for (int m = 0; m < NW; m++) {
-    const int w = min(tid + 256 * m, PP - 1);
+    const int w = min(tid + TB * m, PP - 1);
     w0[m] = w / (P * P);
     w1[m] = (w / P) % P;
     w2[m] = w % P;
@@ -433,20 +447,20 @@ __global__ __launch_bounds__(256) void k_m2l(const int* __restrict__ groupBeg,
     R2 f[NW];
 #pragma unroll
     for (int m = 0; m < NW; m++) {
-      const int w = tid + 256 * m;
+      const int w = tid + TB * m;
       f[m] = w < PP ? Fr[w] : c2<R>(0, 0);
     }
 #pragma unroll
     for (int g = 0; g < G; g++) {
       if (!((mask >> g) & 1)) continue;
       const int code = gCode[(size_t)e * G + g];
-      const R2* Ur = U + (size_t)(code >> 6) * PP;
-      const int h = code & 63;
+      const R2* Ur = U + tableRow<PP>(code);
+      const int z = code & 7;
 #pragma unroll
       for (int m = 0; m < NW; m++) {
-        const int w = tid + 256 * m;
+        const int w = tid + TB * m;
         if (w < PP) {
-          const R2 u = Ur[permIndex<P>(h, w0[m], w1[m], w2[m])];
+          const R2 u = Ur[flipIndex<P>(z, w0[m], w1[m], w2[m])];
           acc[g][m].x += u.x * f[m].x - u.y * f[m].y;
           acc[g][m].y += u.x * f[m].y + u.y * f[m].x;
         }
@@ -462,12 +476,12 @@ __global__ __launch_bounds__(256) void k_m2l(const int* __restrict__ groupBeg,
     __syncthreads();
 #pragma unroll
     for (int m = 0; m < NW; m++) {
-      const int w = tid + 256 * m;
+      const int w = tid + TB * m;
       if (w < PP) J[w] = acc[g][m];
     }
     __syncthreads();
     // axis 0: T1[(k0*P + w1)*P + w2], k0 < L
-    for (int x = tid; x < L * P * P; x += 256) {
+    for (int x = tid; x < L * P * P; x += TB) {
       const int r = x % (P * P), k0 = x / (P * P);
       R2 s = c2<R>(0, 0);
 #pragma unroll
@@ -481,7 +495,7 @@ __global__ __launch_bounds__(256) void k_m2l(const int* __restrict__ groupBeg,
     }
     __syncthreads();
     // axis 1: T2[(k0*L + k1)*P + w2], k1 < L
-    for (int x = tid; x < L * L * P; x += 256) {
+    for (int x = tid; x < L * L * P; x += TB) {
       const int v2 = x % P, k1 = (x / P) % L, k0 = x / (L * P);
       R2 s = c2<R>(0, 0);
 #pragma unroll
@@ -495,7 +509,7 @@ __global__ __launch_bounds__(256) void k_m2l(const int* __restrict__ groupBeg,
     }
     __syncthreads();
     // axis 2 and scaling: L[(k0*L + k1)*L + k2]
-    for (int x = tid; x < LL; x += 256) {
+    for (int x = tid; x < LL; x += TB) {
       const int k2 = x % L, r = x / L;
       R2 s = c2<R>(0, 0);
 #pragma unroll
@@ -515,7 +529,7 @@ __global__ __launch_bounds__(256) void k_m2l(const int* __restrict__ groupBeg,
 // circulant.hpp:95-106: pruned 3-pass DFT of the embedded L^3 corner, in
 // LDS), so only M (L^3) is read per entry and no F store exists.
 template <typename R, int L>
-__global__ __launch_bounds__(256) void k_m2l_fly(const int* __restrict__ groupBeg,
+__global__ __launch_bounds__(M2LShape<L>::TB) void k_m2l_fly(const int* __restrict__ groupBeg,
                                                  const int* __restrict__ groupTarget,
                                                  const int* __restrict__ gSrcSlot,
                                                  const int* __restrict__ gMask,
@@ -527,8 +541,7 @@ __global__ __launch_bounds__(256) void k_m2l_fly(const int* __restrict__ groupBe
                                                  const R* __restrict__ tws, int nItems) {
   typedef typename C2T<R>::T R2;
   constexpr int P = 2 * L - 1, LL = L * L * L, PP = P * P * P;
-  constexpr int NW = (PP + 255) / 256;
-  constexpr int G = kM2LGroup;
+  constexpr int NW = M2LShape<L>::NW, TB = M2LShape<L>::TB, G = M2LShape<L>::G;
   constexpr int NB = L * P * P;  // >= L*L*P
   __shared__ R2 sM[LL], B1[NB], B2[NB], J[PP];
   __shared__ R wc[P], ws[P];
@@ -544,7 +557,7 @@ __global__ __launch_bounds__(256) void k_m2l_fly(const int* __restrict__ groupBe
   int w0[NW], w1[NW], w2[NW];
 #pragma unroll
   for (int m = 0; m < NW; m++) {
-    const int w = min(tid + 256 * m, PP - 1);
+    const int w = min(tid + TB * m, PP - 1);
     w0[m] = w / (P * P);
     w1[m] = (w / P) % P;
     w2[m] = w % P;
@@ -559,10 +572,10 @@ __global__ __launch_bounds__(256) void k_m2l_fly(const int* __restrict__ groupBe
     const R2* Mr = M + (size_t)gSrcSlot[e] * LL;
     const int mask = gMask[e];
     __syncthreads();
-    for (int I = tid; I < LL; I += 256) sM[I] = Mr[I];
+    for (int I = tid; I < LL; I += TB) sM[I] = Mr[I];
     __syncthreads();
     // axis 2: B1[(i0*L + i1)*P + k2]
-    for (int x = tid; x < L * L * P; x += 256) {
+    for (int x = tid; x < L * L * P; x += TB) {
       const int k2 = x % P, r = x / P;
       R2 t = c2<R>(0, 0);
 #pragma unroll
@@ -576,7 +589,7 @@ __global__ __launch_bounds__(256) void k_m2l_fly(const int* __restrict__ groupBe
     }
     __syncthreads();
     // axis 1: B2[(i0*P + k1)*P + k2]
-    for (int x = tid; x < L * P * P; x += 256) {
+    for (int x = tid; x < L * P * P; x += TB) {
       const int k2 = x % P, k1 = (x / P) % P, i0 = x / (P * P);
       R2 t = c2<R>(0, 0);
 #pragma unroll
@@ -593,7 +606,7 @@ __global__ __launch_bounds__(256) void k_m2l_fly(const int* __restrict__ groupBe
     R2 f[NW];
 #pragma unroll
     for (int m = 0; m < NW; m++) {
-      const int w = tid + 256 * m;
+      const int w = tid + TB * m;
       R2 t = c2<R>(0, 0);
       if (w < PP) {
         const int r = w % (P * P);
@@ -611,13 +624,13 @@ __global__ __launch_bounds__(256) void k_m2l_fly(const int* __restrict__ groupBe
     for (int g = 0; g < G; g++) {
       if (!((mask >> g) & 1)) continue;
       const int code = gCode[(size_t)e * G + g];
-      const R2* Ur = U + (size_t)(code >> 6) * PP;
-      const int h = code & 63;
+      const R2* Ur = U + tableRow<PP>(code);
+      const int z = code & 7;
 #pragma unroll
       for (int m = 0; m < NW; m++) {
-        const int w = tid + 256 * m;
+        const int w = tid + TB * m;
         if (w < PP) {
-          const R2 u = Ur[permIndex<P>(h, w0[m], w1[m], w2[m])];
+          const R2 u = Ur[flipIndex<P>(z, w0[m], w1[m], w2[m])];
           acc[g][m].x += u.x * f[m].x - u.y * f[m].y;
           acc[g][m].y += u.x * f[m].y + u.y * f[m].x;
         }
@@ -633,11 +646,11 @@ __global__ __launch_bounds__(256) void k_m2l_fly(const int* __restrict__ groupBe
     __syncthreads();
 #pragma unroll
     for (int m = 0; m < NW; m++) {
-      const int w = tid + 256 * m;
+      const int w = tid + TB * m;
       if (w < PP) J[w] = acc[g][m];
     }
     __syncthreads();
-    for (int x = tid; x < L * P * P; x += 256) {
+    for (int x = tid; x < L * P * P; x += TB) {
       const int r = x % (P * P), k0 = x / (P * P);
       R2 s = c2<R>(0, 0);
 #pragma unroll
@@ -650,7 +663,7 @@ __global__ __launch_bounds__(256) void k_m2l_fly(const int* __restrict__ groupBe
       B1[x] = s;
     }
     __syncthreads();
-    for (int x = tid; x < L * L * P; x += 256) {
+    for (int x = tid; x < L * L * P; x += TB) {
       const int v2 = x % P, k1 = (x / P) % L, k0 = x / (L * P);
       R2 s = c2<R>(0, 0);
 #pragma unroll
@@ -663,7 +676,7 @@ __global__ __launch_bounds__(256) void k_m2l_fly(const int* __restrict__ groupBe
       B2[x] = s;
     }
     __syncthreads();
-    for (int x = tid; x < LL; x += 256) {
+    for (int x = tid; x < LL; x += TB) {
       const int k2 = x % L, r = x / L;
       R2 s = c2<R>(0, 0);
 #pragma unroll
@@ -1049,6 +1062,7 @@ struct Exec {
   int nM2LTargets = 0, nM2LZero = 0;
   int *gBeg = nullptr, *gTarget = nullptr, *gSrcF = nullptr, *gMask = nullptr, *gCode = nullptr;
   int nGroups = 0;
+  int m2lG = 0;
   int nTslots = 0, nSslots = 0;
   P2PItem* items = nullptr;
   int nitems = 0;
@@ -1116,6 +1130,14 @@ struct Exec {
 };
 
 template <typename R>
+static typename C2T<R>::T c2h(double a, double b) {
+  typename C2T<R>::T r;
+  r.x = (R)a;
+  r.y = (R)b;
+  return r;
+}
+
+template <typename R>
 static std::vector<typename C2T<R>::T> toR2(const std::vector<cplx>& v) {
   std::vector<typename C2T<R>::T> o(v.size());
   for (size_t i = 0; i < v.size(); i++) {
@@ -1144,8 +1166,21 @@ static int uploadTyped(Exec& X) {
     a = toR2<R>(X.hSig2);
     CKR(X.upv(d, a));
     X.sig2 = d;
-    a = toR2<R>(X.hTab);
-    CKR(X.upv(d, a));
+    // 6 axis orders per table (see tableRow)
+    static const int kVarSigma[6][3] = {{0, 1, 2}, {1, 0, 2}, {1, 2, 0},
+                                        {0, 2, 1}, {2, 0, 1}, {2, 1, 0}};
+    const int P = X.P, PP = X.PP;
+    const size_t ntab = X.hTab.size() / (size_t)PP;
+    std::vector<R2> u6(ntab * 6 * (size_t)PP);
+    for (size_t t = 0; t < ntab; t++)
+      for (int v = 0; v < 6; v++)
+        for (int i = 0; i < PP; i++) {
+          int u[3] = {i / (P * P), (i / P) % P, i % P};
+          int src = (u[kVarSigma[v][0]] * P + u[kVarSigma[v][1]]) * P + u[kVarSigma[v][2]];
+          const cplx& c = X.hTab[t * PP + src];
+          u6[(t * 6 + v) * PP + i] = c2h<R>(c.real(), c.imag());
+        }
+    CKR(X.upv(d, u6));
     X.U = d;
   }
   {
@@ -1259,6 +1294,7 @@ static int upload(Exec& X, Plan& pl) {
   CKR(X.upv(X.m2lOrder, pl.m2lOrder)); X.nM2LTargets = (int)pl.m2lOrder.size();
   CKR(X.upv(X.m2lZero, pl.m2lZero)); X.nM2LZero = (int)pl.m2lZero.size();
   CKR(X.upv(X.gBeg, pl.m2lGroupBeg)); X.nGroups = (int)pl.m2lGroupBeg.size() - 1;
+  X.m2lG = pl.m2lG;
   CKR(X.upv(X.gTarget, pl.m2lGroupTarget));
   CKR(X.upv(X.gSrcF, pl.m2lGSrcF));
   CKR(X.upv(X.gMask, pl.m2lGMask));
@@ -1374,18 +1410,7 @@ static int runTyped(Exec& X, const double2* qin, double2* pout, int parts, hipSt
   hipLaunchKernelGGL(k_gather<R>, dim3(nblk(X.ns, 256)), dim3(256), 0, s, qin, X.sperm, q, (int)X.ns);
   mark(1);
   const bool far = (parts & 2) != 0, near = (parts & 1) != 0;
-  if (near && X.nitems) {
-    // near field (P2P) on the side stream: VALU-bound, it overlaps the
-    // memory-bound far field; it only writes the per-item partial sums
-    HIPCK(hipEventRecord(X.evFork, s));
-    HIPCK(hipStreamWaitEvent(X.side, X.evFork, 0));
-    HIPCK(hipEventRecord(X.evP2P0, X.side));
-    hipLaunchKernelGGL(k_p2p<R>, itemGrid(X.nitems), dim3(64), 0, X.side, X.items, X.leafCell,
-                       X.T.pbeg, X.T.nprt, X.tx, X.ty, X.tz, X.sx, X.sy, X.sz, (const R2*)q, X.rb,
-                       X.re, X.kappa, X.partial, X.nitems);
-    HIPCK(hipEventRecord(X.evP2P1, X.side));
-    HIPCK(hipEventRecord(X.evJoin, X.side));
-  }
+
   if (far && X.np2m)
     hipLaunchKernelGGL((k_p2m<R, L>), itemGrid(X.np2m), dim3(128), 0, s, X.p2m, X.S, X.lv, X.sx,
                        X.sy, X.sz, (const R2*)q, X.dirs, sig2, oc, X.kappa, ML, X.np2m);
@@ -1400,14 +1425,33 @@ static int runTyped(Exec& X, const double2* qin, double2* pout, int parts, hipSt
     hipLaunchKernelGGL((k_m2f<R, L>), itemGrid(X.nm2f), dim3(256), 0, s, X.m2fSlot, (const R2*)ML,
                        F, oc.twc, oc.tws, X.nm2f);
   mark(4);
+  if (near && X.nitems) {
+    // near field (P2P) on the side stream, forked here so that the VALU-bound
+    // P2P shares the GPU with the memory-bound M2L; it only writes the
+    // per-item partial sums (q was gathered before)
+    HIPCK(hipEventRecord(X.evFork, s));
+    HIPCK(hipStreamWaitEvent(X.side, X.evFork, 0));
+    HIPCK(hipEventRecord(X.evP2P0, X.side));
+    hipLaunchKernelGGL(k_p2p<R>, itemGrid(X.nitems), dim3(64), 0, X.side, X.items, X.leafCell,
+                       X.T.pbeg, X.T.nprt, X.tx, X.ty, X.tz, X.sx, X.sy, X.sz, (const R2*)q, X.rb,
+                       X.re, X.kappa, X.partial, X.nitems);
+    HIPCK(hipEventRecord(X.evP2P1, X.side));
+    HIPCK(hipEventRecord(X.evJoin, X.side));
+  }
   // L (aliasing M) is written for every target slot here
+  if (X.m2lG != M2LShape<L>::G) {
+    g_err = "internal: M2L group size mismatch";
+    return ERR_STATE;
+  }
   if (far && X.nGroups) {
     if (X.storeF)
-      hipLaunchKernelGGL((k_m2l<R, L>), itemGrid(X.nGroups), dim3(256), 0, s, X.gBeg, X.gTarget,
+      hipLaunchKernelGGL((k_m2l<R, L>), itemGrid(X.nGroups), dim3(M2LShape<L>::TB), 0, s, X.gBeg,
+                         X.gTarget,
                          X.gSrcF, X.gMask, X.gCode, (const R2*)F, (const R2*)X.U, Lb, oc.twc,
                          oc.tws, X.nGroups);
     else
-      hipLaunchKernelGGL((k_m2l_fly<R, L>), itemGrid(X.nGroups), dim3(256), 0, s, X.gBeg,
+      hipLaunchKernelGGL((k_m2l_fly<R, L>), itemGrid(X.nGroups), dim3(M2LShape<L>::TB), 0, s,
+                         X.gBeg,
                          X.gTarget, X.gSrcSlot, X.gMask, X.gCode, (const R2*)ML,
                          (const R2*)X.U, Lb, oc.twc, oc.tws, X.nGroups);
   }
@@ -1588,7 +1632,7 @@ int defmm_stage_times(defmm_plan plan, double* ms, int n) {
     float t = 0.f;
     hipEvent_t a = X.ev[k], b = X.ev[k + 1];
     if (k == 7) {
-      // the near field ran on the side stream, overlapping stages 1..6
+      // the near field ran on the side stream, overlapping stages 4..6
       if (!X.p2pRecorded) { ms[k] = 0.; continue; }
       a = X.evP2P0;
       b = X.evP2P1;

@@ -754,12 +754,13 @@ int buildPlan(Plan& pl, const double* src, int64_t ns, const double* trg, int64_
     });
     pl.m2lOrder.swap(ord);
   }
-  // Groups of up to kM2LGroup consecutive targets of one level and direction
+  // Groups of up to m2lGroupSize(L) consecutive targets of one level and direction
   // (neighbouring cells): a group's entries are the union of its targets'
   // source rows, each with the mask of targets that read it and their
   // (table, permutation) codes, so that a source row is loaded once per group.
   {
-    const int G = kM2LGroup;
+    const int G = m2lGroupSize(L);
+    pl.m2lG = G;
     pl.m2lGroupBeg.push_back(0);
     std::vector<std::array<int, 3>> tmp;  // srcF, target in group, tabHash
     size_t i = 0;
@@ -767,7 +768,7 @@ int buildPlan(Plan& pl, const double* src, int64_t ns, const double* trg, int64_
       int t0 = pl.m2lOrder[i];
       int l0 = pl.T.lvl[pl.T.slotCell[t0]], d0 = pl.T.slotDir[t0];
       int ng = 0;
-      int tg[kM2LGroup];
+      int tg[kM2LGroupMax];
       while (ng < G && i < pl.m2lOrder.size()) {
         int t = pl.m2lOrder[i];
         if (pl.T.lvl[pl.T.slotCell[t]] != l0 || pl.T.slotDir[t] != d0) break;
@@ -787,7 +788,7 @@ int buildPlan(Plan& pl, const double* src, int64_t ns, const double* trg, int64_
       while (k < tmp.size()) {
         int src = tmp[k][0];
         int mask = 0;
-        int codes[kM2LGroup] = {0};
+        int codes[kM2LGroupMax] = {0};
         size_t k2 = k;
         // a target reads a source row at most once per group entry; a repeat
         // (the same pair listed twice) starts a new entry
@@ -939,17 +940,19 @@ int validatePlan(const Plan& pl, std::string& msg) {
     if (v < 0 || v >= pl.T.nslots) return bad("m2l zero list");
   {
     size_t ng = pl.m2lGroupBeg.size() - 1;
-    if (pl.m2lGroupTarget.size() != ng * kM2LGroup) return bad("m2l group targets");
+    const int GG = pl.m2lG;
+    if (GG < 1 || GG > kM2LGroupMax) return bad("m2l group size");
+    if (pl.m2lGroupTarget.size() != ng * GG) return bad("m2l group targets");
     if (pl.m2lGroupBeg.back() != (int)pl.m2lGSrcF.size() ||
         pl.m2lGMask.size() != pl.m2lGSrcF.size() ||
-        pl.m2lGCode.size() != pl.m2lGSrcF.size() * kM2LGroup)
+        pl.m2lGCode.size() != pl.m2lGSrcF.size() * GG)
       return bad("m2l group csr");
     std::vector<char> seen(pl.T.nslots, 0);
     for (size_t g = 0; g < ng; g++) {
       if (pl.m2lGroupBeg[g + 1] < pl.m2lGroupBeg[g]) return bad("m2l group order");
       int have = 0;
-      for (int k = 0; k < kM2LGroup; k++) {
-        int t = pl.m2lGroupTarget[g * kM2LGroup + k];
+      for (int k = 0; k < GG; k++) {
+        int t = pl.m2lGroupTarget[g * GG + k];
         if (t < -1 || t >= pl.T.nslots) return bad("m2l group target");
         if (t >= 0) {
           if (seen[t]) return bad("m2l target in two groups");
@@ -961,9 +964,9 @@ int validatePlan(const Plan& pl, std::string& msg) {
         if (pl.m2lGSrcF[e] < 0 || pl.m2lGSrcF[e] >= (int)pl.m2fSlot.size()) return bad("m2l group source");
         int m = pl.m2lGMask[e];
         if (m <= 0 || (m & ~have)) return bad("m2l group mask");
-        for (int k = 0; k < kM2LGroup; k++) {
+        for (int k = 0; k < GG; k++) {
           if (!((m >> k) & 1)) continue;
-          int c = pl.m2lGCode[(size_t)e * kM2LGroup + k];
+          int c = pl.m2lGCode[(size_t)e * GG + k];
           if ((c >> 6) < 0 || (c >> 6) >= pl.ntab) return bad("m2l group table");
         }
       }

@@ -16,7 +16,10 @@ namespace dfm {
 
 typedef std::complex<double> cplx;
 
-constexpr int kM2LGroup = 4;  // target slots per M2L work group
+// Target slots per M2L work group: a thread holds one accumulator per group
+// target and frequency it owns.
+constexpr int kM2LGroupMax = 8;
+inline int m2lGroupSize(int /*L*/) { return kM2LGroupMax / 2; }
 
 // Error codes returned through the C ABI.
 enum {
@@ -119,9 +122,10 @@ struct Plan {
   std::vector<int> m2lHash;     // permutation
   std::vector<int> m2lOrder;    // target slots with M2L entries, in processing order
   std::vector<int> m2lZero;     // target slots without M2L entries (L = 0)
-  // grouped M2L: per group kM2LGroup target slots (-1 padded); per entry a
-  // source F row, the mask of the group's targets reading it and their codes
-  // (table id << 6 | permutation hash), kM2LGroup per entry
+  // grouped M2L: per group m2lG target slots (-1 padded); per entry a source
+  // F row, the mask of the group's targets reading it and their codes
+  // (table id << 6 | permutation hash), m2lG per entry
+  int m2lG = 4;
   std::vector<int> m2lGroupBeg, m2lGroupTarget, m2lGSrcF, m2lGMask, m2lGCode;
   // P2P, CSR by target leaf: source particle ranges [b, e)
   std::vector<int> p2pLeaf;     // target leaf cells that receive near field
