@@ -72,8 +72,12 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--samples", type=int, default=2000)
-    ap.add_argument("--both-m2l", action="store_true",
-                    help="also time the M2L variant that recomputes Fourier rows on the fly")
+    ap.add_argument("--m2l", default="auto", choices=["auto", "batched", "stored", "fly"],
+                    help="M2L variant of the timed pass (auto: the library's choice)")
+    ap.add_argument("--batch-rows", type=int, default=None,
+                    help="workspace rows of the batched M2L (library default if unset)")
+    ap.add_argument("--compare-m2l", action="store_true",
+                    help="also time the other M2L variants (FP64)")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="after timing, copy this many GiB device-to-device (a kernel of known "
                          "traffic, to scale rocprof byte counters)")
@@ -98,6 +102,10 @@ def main():
     t0 = time.time()
     A.prcmpt(L, ncrit, kappa)
     setup_s = time.time() - t0
+    modes = {"batched": dm.M2L_BATCHED, "stored": dm.M2L_STORED, "fly": dm.M2L_ON_THE_FLY}
+    if args.m2l == "auto":
+        args.m2l = {v: k for k, v in modes.items()}[A.m2l_mode()]
+    A.set_m2l(modes[args.m2l], batch_rows=args.batch_rows)
     st = A.stats()
     log(f"setup {setup_s:.1f} s, device {A.device_bytes() / 1e9:.1f} GB, stats {st}")
 
@@ -130,26 +138,38 @@ def main():
         log(f"[{tag}] {ms:.2f} ms/step, rel L2 {err2:.3e}, Linf {errinf:.3e}, stages {avg}")
         return ms, err2, errinf, avg
 
-    ms64, e64, einf64, st64 = run("fp64")
+    ms64, e64, einf64, st64 = run(f"fp64 m2l={args.m2l}")
     alt = None
-    if args.both_m2l:
-        A.set_m2l(False)
-        msa, ea, einfa, sta = run("fp64 m2l-on-the-fly")
-        alt = {"ms_per_step": msa, "rel_l2_err": ea, "stage_ms": sta}
-        A.set_m2l(True)
+    if args.compare_m2l:
+        alt = {}
+        for name in modes:
+            if name == args.m2l:
+                continue
+            A.set_m2l(modes[name])
+            msa, ea, _, sta = run(f"fp64 m2l={name}")
+            alt[name] = {"ms_per_step": msa, "rel_l2_err": ea, "stage_ms": sta,
+                         "device_gb": A.device_bytes() / 1e9}
+        A.set_m2l(modes[args.m2l])
     value = n / (ms64 * 1e-3) / 1e6
 
     # dominant kernel of the FP64 pass and its roofline
     dom = max(["m2l_f2l", "p2p"], key=lambda k: st64[k])
     if dom == "m2l_f2l":
-        PP = (2 * L - 1) ** 3
-        LL = L ** 3
-        alg = st["m2l_entries"] * (PP * 16 + 8) + st["trg_slots"] * LL * 16
+        # Hadamard bytes: (M2L pairs + target slots) x PP complex128, tables
+        # counted once (cache-resident, negligible); the batched variant runs
+        # M2F inside this stage: + F rows x (LL + PP) complex128
+        PP, LL = (2 * L - 1) ** 3, L ** 3
+        alg = (st["m2l_entries"] + st["m2l_targets"]) * PP * 16
+        kernel = "k_m2l (M2L Hadamard + F2L)"
+        if args.m2l == "batched":
+            alg += st["m2l_batch_rows"] * (LL + PP) * 16
+            kernel = (f"k_m2f + k_m2l per batch ({st['m2l_batches']} batches of <= "
+                      f"{args.batch_rows or 16384} F rows): M2F, M2L Hadamard, F2L")
         achieved = alg / (st64[dom] * 1e-3) / 1e9
-        roof = {"kernel": "k_m2l (fused M2L+F2L)", "bound": "hbm", "achieved": achieved,
+        roof = {"kernel": kernel, "bound": "hbm", "achieved": achieved,
                 "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
-                "traffic": read_traffic("m2l", args.workload), "algorithmic_bytes": alg,
-                "avg_ms": st64[dom]}
+                "traffic": read_traffic(f"m2l_{args.m2l}", args.workload),
+                "algorithmic_bytes": alg, "avg_ms": st64[dom]}
     else:
         flops = st["p2p_pairs"] * P2P_FLOPS_PER_PAIR
         achieved = flops / (st64[dom] * 1e-3) / 1e12
@@ -199,9 +219,10 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "mixed": mixed,
-        "fp64_m2l_on_the_fly": alt,
+        "m2l_variants_fp64": alt,
         "plan": {k: st[k] for k in ("p2p_pairs", "hf_m2l", "src_slots", "trg_slots",
-                                      "m2f_slots", "m2l_entries", "p2p_items")},
+                                      "m2f_slots", "m2l_entries", "m2l_group_entries",
+                                      "m2l_batches", "m2l_batch_rows", "p2p_items")},
     }
     print(json.dumps(out), flush=True)
 

@@ -25,12 +25,17 @@ __all__ = ["IBFMM_Mat", "gemv", "DefmmError", "lib", "FP64", "MIXED", "STAT_NAME
 
 FP64 = 0
 MIXED = 1
+# M2L variants (defmm_amd.h: DEFMM_M2L_*)
+M2L_ON_THE_FLY = 0
+M2L_STORED = 1
+M2L_BATCHED = 2
 
 STAT_NAMES = [
     "p2p_calls", "p2p_pairs", "hf_m2l", "lf_m2l", "src_slots", "trg_slots", "hf_tables",
     "lf_tables", "hf_leaves", "lf_leaves", "hf_m2m", "lf2hf_m2m", "lf_m2m", "hf_l2p",
     "lf_l2p", "hf_l2l", "hf2lf_l2l", "lf_l2l", "nlvl", "cpt", "max_stage", "ncells_t",
-    "ncells_s", "p2p_items", "m2f_slots", "m2l_entries", "m2l_targets",
+    "ncells_s", "p2p_items", "m2f_slots", "m2l_entries", "m2l_targets", "m2l_groups",
+    "m2l_group_entries", "m2l_batches", "m2l_batch_rows",
 ]
 STAGE_NAMES = ["gather", "p2m", "m2m", "m2f", "m2l_f2l", "l2l", "l2p", "p2p", "scatter"]
 
@@ -74,6 +79,8 @@ def lib():
                                          ctypes.c_int]
     L.defmm_plan_set_precision.argtypes = [P, ctypes.c_int]
     L.defmm_plan_set_m2l.argtypes = [P, ctypes.c_int]
+    L.defmm_plan_set_m2l_batch_rows.argtypes = [P, ctypes.c_int]
+    L.defmm_plan_get_m2l.argtypes = [P, ctypes.POINTER(ctypes.c_int)]
     L.defmm_direct.argtypes = [D, ctypes.c_int64, D, D, ctypes.c_int64, ctypes.c_double, D]
     _lib = L
     return L
@@ -209,10 +216,20 @@ class IBFMM_Mat:
         _check(lib().defmm_plan_set_precision(self._plan, int(precision)))
         self.precision = precision
 
-    def set_m2l(self, store_fourier):
-        """M2L from stored Fourier rows (True) or rows computed on the fly (False)."""
+    def set_m2l(self, mode, batch_rows=None):
+        """M2L variant: M2L_BATCHED (default), M2L_STORED or M2L_ON_THE_FLY;
+        batch_rows sets the batched variant's workspace rows."""
         self._need_plan()
-        _check(lib().defmm_plan_set_m2l(self._plan, int(bool(store_fourier))))
+        if batch_rows is not None:
+            _check(lib().defmm_plan_set_m2l_batch_rows(self._plan, int(batch_rows)))
+        _check(lib().defmm_plan_set_m2l(self._plan, int(mode)))
+
+    def m2l_mode(self):
+        """The plan's current M2L variant (M2L_STORED, M2L_BATCHED, M2L_ON_THE_FLY)."""
+        self._need_plan()
+        m = ctypes.c_int()
+        _check(lib().defmm_plan_get_m2l(self._plan, ctypes.byref(m)))
+        return m.value
 
     def stats(self):
         self._need_plan()

@@ -372,17 +372,21 @@ __global__ __launch_bounds__(256) void k_m2f(const int* __restrict__ m2fSlot,
 // U_t[perm_h(w)] = V[flip_z(w)] with h = z + 8 s (permutations.hpp:127-150):
 // component j of w is replaced by P - w_j when bit j of z is set and w_j != 0.
 // Only sign flips remain, which keep a row of w contiguous (reversed), so the
-// gather reads whole cache lines.
-__constant__ int c_variant[8] = {0, 1, 0, 2, 3, 0, 4, 5};
+// gather reads whole cache lines.  The device code of a (table, h) pair is
+// ((t * 6 + v(s)) << 3) | z: the row of the table variant and the flips.
+static const int kVariantOfSigma[8] = {0, 1, 0, 2, 3, 0, 4, 5};
 
-// M2L block shape for order L: 256 threads, NW frequencies per thread, G
-// target slots per group (measured faster at sphere 1M / 128 wavelengths than
-// 768-thread blocks with one frequency per thread, or groups of 8)
+// M2L block shape for order L: NW frequencies per thread (at most 3 where a
+// block of <= 1024 threads covers PP, fewer when 256 threads suffice), TB
+// threads, G target slots per group.  A thread keeps NW F values, G x NW
+// table values and G x NW accumulators in registers.
 template <int L>
 struct M2LShape {
   static constexpr int PP = (2 * L - 1) * (2 * L - 1) * (2 * L - 1);
-  static constexpr int TB = 256;
-  static constexpr int NW = (PP + TB - 1) / TB;
+  static constexpr int NWmin = (PP + 1023) / 1024;
+  static constexpr int NWpref = (PP + 255) / 256 < 3 ? (PP + 255) / 256 : 3;
+  static constexpr int NW = NWmin > NWpref ? NWmin : NWpref;
+  static constexpr int TB = ((PP + NW - 1) / NW + 63) / 64 * 64;
   static constexpr int G = kM2LGroupMax / 2;
 };
 
@@ -395,7 +399,7 @@ __device__ __forceinline__ int flipIndex(int z, int w0, int w1, int w2) {
 }
 template <int PP>
 __device__ __forceinline__ size_t tableRow(int code) {
-  return ((size_t)(code >> 6) * 6 + (size_t)c_variant[(code >> 3) & 7]) * PP;
+  return (size_t)(code >> 3) * PP;
 }
 
 template <typename R, int L>
@@ -427,10 +431,13 @@ __global__ __launch_bounds__(M2LShape<L>::TB) void k_m2l(const int* __restrict__
     wc[tid] = twc[tid];
     ws[tid] = tws[tid];
   }
-  int w0[NW], w1[NW], w2[NW];
+  // frequencies of this thread, clamped to PP - 1: every load is in bounds
+  // and unconditional, and the accumulators of w >= PP are never stored
+  int wl[NW], w0[NW], w1[NW], w2[NW];
 #pragma unroll
   for (int m = 0; m < NW; m++) {
     const int w = min(tid + TB * m, PP - 1);
+    wl[m] = w;
     w0[m] = w / (P * P);
     w1[m] = (w / P) % P;
     w2[m] = w % P;
@@ -444,26 +451,32 @@ __global__ __launch_bounds__(M2LShape<L>::TB) void k_m2l(const int* __restrict__
   for (int e = e0; e < e1; e++) {
     const R2* Fr = F + (size_t)gSrcF[e] * PP;
     const int mask = gMask[e];
-    R2 f[NW];
+    int code[G];
 #pragma unroll
-    for (int m = 0; m < NW; m++) {
-      const int w = tid + TB * m;
-      f[m] = w < PP ? Fr[w] : c2<R>(0, 0);
+    for (int g = 0; g < G; g++) code[g] = gCode[(size_t)e * G + g];
+    // issue all of the entry's loads before the first use
+    R2 f[NW], u[G][NW];
+#pragma unroll
+    for (int m = 0; m < NW; m++) f[m] = Fr[wl[m]];
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      const R2* Ur = U + tableRow<PP>(code[g]);
+      const int z = code[g] & 7;
+      if ((mask >> g) & 1) {
+#pragma unroll
+        for (int m = 0; m < NW; m++) u[g][m] = Ur[flipIndex<P>(z, w0[m], w1[m], w2[m])];
+      } else {
+#pragma unroll
+        for (int m = 0; m < NW; m++) u[g][m] = c2<R>(0, 0);
+      }
     }
 #pragma unroll
     for (int g = 0; g < G; g++) {
       if (!((mask >> g) & 1)) continue;
-      const int code = gCode[(size_t)e * G + g];
-      const R2* Ur = U + tableRow<PP>(code);
-      const int z = code & 7;
 #pragma unroll
       for (int m = 0; m < NW; m++) {
-        const int w = tid + TB * m;
-        if (w < PP) {
-          const R2 u = Ur[flipIndex<P>(z, w0[m], w1[m], w2[m])];
-          acc[g][m].x += u.x * f[m].x - u.y * f[m].y;
-          acc[g][m].y += u.x * f[m].y + u.y * f[m].x;
-        }
+        acc[g][m].x += u[g][m].x * f[m].x - u[g][m].y * f[m].y;
+        acc[g][m].y += u[g][m].x * f[m].y + u[g][m].y * f[m].x;
       }
     }
   }
@@ -571,6 +584,22 @@ __global__ __launch_bounds__(M2LShape<L>::TB) void k_m2l_fly(const int* __restri
   for (int e = e0; e < e1; e++) {
     const R2* Mr = M + (size_t)gSrcSlot[e] * LL;
     const int mask = gMask[e];
+    // the table values are loaded first: they arrive while the F row is
+    // computed in LDS
+    R2 u[G][NW];
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      const int code = gCode[(size_t)e * G + g];
+      const R2* Ur = U + tableRow<PP>(code);
+      const int z = code & 7;
+      if ((mask >> g) & 1) {
+#pragma unroll
+        for (int m = 0; m < NW; m++) u[g][m] = Ur[flipIndex<P>(z, w0[m], w1[m], w2[m])];
+      } else {
+#pragma unroll
+        for (int m = 0; m < NW; m++) u[g][m] = c2<R>(0, 0);
+      }
+    }
     __syncthreads();
     for (int I = tid; I < LL; I += TB) sM[I] = Mr[I];
     __syncthreads();
@@ -623,17 +652,10 @@ __global__ __launch_bounds__(M2LShape<L>::TB) void k_m2l_fly(const int* __restri
 #pragma unroll
     for (int g = 0; g < G; g++) {
       if (!((mask >> g) & 1)) continue;
-      const int code = gCode[(size_t)e * G + g];
-      const R2* Ur = U + tableRow<PP>(code);
-      const int z = code & 7;
 #pragma unroll
       for (int m = 0; m < NW; m++) {
-        const int w = tid + TB * m;
-        if (w < PP) {
-          const R2 u = Ur[flipIndex<P>(z, w0[m], w1[m], w2[m])];
-          acc[g][m].x += u.x * f[m].x - u.y * f[m].y;
-          acc[g][m].y += u.x * f[m].y + u.y * f[m].x;
-        }
+        acc[g][m].x += u[g][m].x * f[m].x - u[g][m].y * f[m].y;
+        acc[g][m].y += u[g][m].x * f[m].y + u[g][m].y * f[m].x;
       }
     }
   }
@@ -1038,11 +1060,8 @@ struct Exec {
   std::vector<void*> mem;
   int64_t bytes = 0;
   hipStream_t own = nullptr;
-  hipStream_t side = nullptr;     // near field runs here, overlapping the far field
   hipEvent_t ev[DEFMM_NSTAGES + 1] = {};
-  hipEvent_t evFork = nullptr, evJoin = nullptr, evP2P0 = nullptr, evP2P1 = nullptr;
   bool evRecorded = false;
-  bool p2pRecorded = false;
 
   double *sx = nullptr, *sy = nullptr, *sz = nullptr, *tx = nullptr, *ty = nullptr, *tz = nullptr;
   int *sperm = nullptr, *tperm = nullptr;
@@ -1060,7 +1079,10 @@ struct Exec {
   int *m2lBeg = nullptr, *m2lSrcF = nullptr, *m2lTabHash = nullptr;
   int *m2lOrder = nullptr, *m2lZero = nullptr;
   int nM2LTargets = 0, nM2LZero = 0;
-  int *gBeg = nullptr, *gTarget = nullptr, *gSrcF = nullptr, *gMask = nullptr, *gCode = nullptr;
+  int *gBeg = nullptr, *gTarget = nullptr, *gMask = nullptr, *gCode = nullptr;
+  // per group entry, the row it reads: a global F row (stored), a source
+  // slot (on the fly) or a workspace row of its batch (batched)
+  int* gSrc = nullptr;
   int nGroups = 0;
   int m2lG = 0;
   int nTslots = 0, nSslots = 0;
@@ -1071,8 +1093,14 @@ struct Exec {
   double2* partial = nullptr;
   void *q = nullptr, *p = nullptr, *ML = nullptr, *F = nullptr;
   void* Lb = nullptr;        // local expansions (aliases ML when F is stored)
-  bool storeF = true;        // M2L reads a stored F (else: F from M on the fly)
-  int* gSrcSlot = nullptr;   // grouped M2L entries: source slot (for the M rows)
+  int m2lMode = DEFMM_M2L_STORED;
+  int batchRows = 16384;     // requested workspace rows of the batched M2L
+  int wsRows = 0;            // rows of the workspace in use
+  int* bRows = nullptr;      // batched M2L: source slots of the batches' rows
+  std::vector<int> hBatchGroupBeg, hBatchRowBeg;
+  // host copies of the grouped entries' global F rows, the F-row -> source
+  // slot map and the group bounds (the mode-dependent lists are built from them)
+  std::vector<int> hGSrcF, hM2fSlot, hGroupBeg;
   double2 *qio = nullptr, *pio = nullptr;
   double stageMs[DEFMM_NSTAGES] = {};
   // host copies needed to re-upload the typed buffers in another precision
@@ -1094,10 +1122,7 @@ struct Exec {
     for (void* m : mem) hipFree(m);
     for (auto& e : ev)
       if (e) hipEventDestroy(e);
-    for (auto e : {evFork, evJoin, evP2P0, evP2P1})
-      if (e) hipEventDestroy(e);
     if (own) hipStreamDestroy(own);
-    if (side) hipStreamDestroy(side);
   }
 
   template <class X>
@@ -1153,6 +1178,67 @@ static std::vector<typename C2T<R>::T> toR2(const std::vector<cplx>& v) {
     if (r_ != OK) return r_; \
   } while (0)
 
+// Batched M2L: cut the groups, in processing order, into batches whose
+// distinct source F rows number at most wsRows; per batch, the rows (source
+// slots to transform) in first-use order and, per group entry, its row in
+// the batch.  A group is never split.
+static int buildBatches(Exec& X, std::vector<int>& local, std::vector<int>& rows) {
+  const std::vector<int>& gb = X.hGroupBeg;
+  const int ng = (int)gb.size() - 1;
+  int maxRows = 1;
+  for (int g = 0; g < ng; g++) maxRows = std::max(maxRows, gb[g + 1] - gb[g]);
+  X.wsRows = std::max(X.batchRows, maxRows);
+  const int W = X.wsRows;
+  local.assign(X.hGSrcF.size(), -1);
+  rows.clear();
+  X.hBatchGroupBeg.assign(1, 0);
+  X.hBatchRowBeg.assign(1, 0);
+  // inBatch[r]: last batch that holds F row r, rowOf[r]: its row there;
+  // stamp[r]: last group that counted r
+  std::vector<int> inBatch(X.hM2fSlot.size(), -1), rowOf(X.hM2fSlot.size(), -1),
+      stamp(X.hM2fSlot.size(), -1);
+  int b = 0, nIn = 0;
+  for (int g = 0; g < ng; g++) {
+    int fresh = 0;
+    for (int e = gb[g]; e < gb[g + 1]; e++) {
+      int r = X.hGSrcF[e];
+      if (inBatch[r] != b && stamp[r] != g) { stamp[r] = g; fresh++; }
+    }
+    if (nIn > 0 && nIn + fresh > W) {
+      X.hBatchGroupBeg.push_back(g);
+      X.hBatchRowBeg.push_back((int)rows.size());
+      b++;
+      nIn = 0;
+    }
+    for (int e = gb[g]; e < gb[g + 1]; e++) {
+      int r = X.hGSrcF[e];
+      if (inBatch[r] != b) {
+        inBatch[r] = b;
+        rowOf[r] = nIn++;
+        rows.push_back(X.hM2fSlot[r]);
+      }
+      local[e] = rowOf[r];
+    }
+  }
+  if (ng > 0) {
+    X.hBatchGroupBeg.push_back(ng);
+    X.hBatchRowBeg.push_back((int)rows.size());
+  }
+  // checks: every entry has a row inside its batch, every batch fits
+  for (size_t k = 0; k + 1 < X.hBatchGroupBeg.size(); k++) {
+    int r0 = X.hBatchRowBeg[k], r1 = X.hBatchRowBeg[k + 1];
+    if (r1 - r0 > W || r1 < r0) { g_err = "internal: M2L batch exceeds its workspace"; return ERR_STATE; }
+    for (int e = gb[X.hBatchGroupBeg[k]]; e < gb[X.hBatchGroupBeg[k + 1]]; e++)
+      if (local[e] < 0 || local[e] >= r1 - r0 || rows[r0 + local[e]] != X.hM2fSlot[X.hGSrcF[e]]) {
+        g_err = "internal: M2L batch row out of range";
+        return ERR_STATE;
+      }
+  }
+  X.st.m2l_batches = (int64_t)X.hBatchGroupBeg.size() - 1;
+  X.st.m2l_batch_rows = (int64_t)rows.size();
+  return OK;
+}
+
 template <typename R>
 static int uploadTyped(Exec& X) {
   typedef typename C2T<R>::T R2;
@@ -1205,17 +1291,29 @@ static int uploadTyped(Exec& X) {
     CKR(X.upv(d, c));
     X.consts = d;
   }
-  if (X.storeF) {
+  if (X.m2lMode == DEFMM_M2L_STORED) {
     // M is dead after M2F and every target slot's L is written by M2L or
     // zeroed: one buffer holds M, then L
     CKR(X.alloc(X.ML, X.nMLslots * X.LL * sizeof(R2)));
-    CKR(X.alloc(X.F, (size_t)X.nm2f * X.PP * sizeof(R2)));
+    CKR(X.alloc(X.F, (size_t)std::max(1, X.nm2f) * X.PP * sizeof(R2)));
     X.Lb = X.ML;
+    CKR(X.upv(X.gSrc, X.hGSrcF));
   } else {
-    // M2L reads M while writing L: two buffers, no F
+    // M2L reads M while writing L: two buffers
     CKR(X.alloc(X.ML, (size_t)std::max(1, X.nSslots) * X.LL * sizeof(R2)));
     CKR(X.alloc(X.Lb, (size_t)std::max(1, X.nTslots) * X.LL * sizeof(R2)));
     X.F = nullptr;
+    if (X.m2lMode == DEFMM_M2L_ON_THE_FLY) {
+      std::vector<int> ss(X.hGSrcF.size());
+      for (size_t e = 0; e < ss.size(); e++) ss[e] = X.hM2fSlot[X.hGSrcF[e]];
+      CKR(X.upv(X.gSrc, ss));
+    } else {
+      std::vector<int> local, rows;
+      CKR(buildBatches(X, local, rows));
+      CKR(X.upv(X.gSrc, local));
+      CKR(X.upv(X.bRows, rows));
+      CKR(X.alloc(X.F, (size_t)X.wsRows * X.PP * sizeof(R2)));
+    }
   }
   CKR(X.alloc(X.q, (size_t)X.ns * sizeof(R2)));
   CKR(X.alloc(X.p, (size_t)X.nt * sizeof(R2)));
@@ -1296,14 +1394,19 @@ static int upload(Exec& X, Plan& pl) {
   CKR(X.upv(X.gBeg, pl.m2lGroupBeg)); X.nGroups = (int)pl.m2lGroupBeg.size() - 1;
   X.m2lG = pl.m2lG;
   CKR(X.upv(X.gTarget, pl.m2lGroupTarget));
-  CKR(X.upv(X.gSrcF, pl.m2lGSrcF));
   CKR(X.upv(X.gMask, pl.m2lGMask));
-  CKR(X.upv(X.gCode, pl.m2lGCode));
   {
-    std::vector<int> ss(pl.m2lGSrcF.size());
-    for (size_t e = 0; e < ss.size(); e++) ss[e] = pl.m2fSlot[pl.m2lGSrcF[e]];
-    CKR(X.upv(X.gSrcSlot, ss));
+    // (table id << 6 | h) -> (table variant row << 3 | flips)
+    std::vector<int> dc(pl.m2lGCode.size());
+    for (size_t k = 0; k < dc.size(); k++) {
+      const int c = pl.m2lGCode[k];
+      dc[k] = (((c >> 6) * 6 + kVariantOfSigma[(c >> 3) & 7]) << 3) | (c & 7);
+    }
+    CKR(X.upv(X.gCode, dc));
   }
+  X.hGSrcF.swap(pl.m2lGSrcF);
+  X.hM2fSlot = pl.m2fSlot;
+  X.hGroupBeg = pl.m2lGroupBeg;
   {
     std::vector<int> th(pl.m2lTabId.size());
     for (size_t k = 0; k < th.size(); k++) th[k] = (pl.m2lTabId[k] << 6) | pl.m2lHash[k];
@@ -1378,6 +1481,20 @@ static int upload(Exec& X, Plan& pl) {
   X.hTab.swap(pl.m2lTab);
   for (int s = 0; s < 2; s++) { X.hA[s] = pl.A[s]; X.hB[s] = pl.B[s]; }
   X.nMLslots = (size_t)std::max(pl.S.nslots, pl.T.nslots);
+  // default M2L variant: stored F rows (one M2F and one M2L launch, the
+  // faster variant) when they fit in 90% of the free device memory with the
+  // shared M/L store, else cache-sized batches (separate M and L stores)
+  {
+    size_t freeB = 0, totalB = 0;
+    if (hipMemGetInfo(&freeB, &totalB) != hipSuccess) {
+      g_err = "hipMemGetInfo failed";
+      return ERR_HIP;
+    }
+    const size_t r2 = X.prec == DEFMM_FP64 ? 16 : 8;
+    const size_t storedB = ((size_t)X.nm2f * X.PP + X.nMLslots * X.LL) * r2 +
+                           (size_t)X.hGSrcF.size() * sizeof(int) + ((size_t)X.ns + X.nt) * r2;
+    X.m2lMode = (double)storedB < 0.9 * (double)freeB ? DEFMM_M2L_STORED : DEFMM_M2L_BATCHED;
+  }
   return uploadPrecision(X);
 }
 
@@ -1421,44 +1538,46 @@ static int runTyped(Exec& X, const double2* qin, double2* pout, int parts, hipSt
                          t.sgn, t.sigIdx, t.edv, (const R2*)ML, ML, sig, sig2, oc.A[0], oc.A[1], 0,
                          t.n);
   mark(3);
-  if (far && X.nm2f && X.storeF)
+  const bool stored = X.m2lMode == DEFMM_M2L_STORED;
+  if (far && X.nm2f && stored)
     hipLaunchKernelGGL((k_m2f<R, L>), itemGrid(X.nm2f), dim3(256), 0, s, X.m2fSlot, (const R2*)ML,
                        F, oc.twc, oc.tws, X.nm2f);
   mark(4);
-  if (near && X.nitems) {
-    // near field (P2P) on the side stream, forked here so that the VALU-bound
-    // P2P shares the GPU with the memory-bound M2L; it only writes the
-    // per-item partial sums (q was gathered before)
-    HIPCK(hipEventRecord(X.evFork, s));
-    HIPCK(hipStreamWaitEvent(X.side, X.evFork, 0));
-    HIPCK(hipEventRecord(X.evP2P0, X.side));
-    hipLaunchKernelGGL(k_p2p<R>, itemGrid(X.nitems), dim3(64), 0, X.side, X.items, X.leafCell,
-                       X.T.pbeg, X.T.nprt, X.tx, X.ty, X.tz, X.sx, X.sy, X.sz, (const R2*)q, X.rb,
-                       X.re, X.kappa, X.partial, X.nitems);
-    HIPCK(hipEventRecord(X.evP2P1, X.side));
-    HIPCK(hipEventRecord(X.evJoin, X.side));
-  }
-  // L (aliasing M) is written for every target slot here
   if (X.m2lG != M2LShape<L>::G) {
     g_err = "internal: M2L group size mismatch";
     return ERR_STATE;
   }
   if (far && X.nGroups) {
-    if (X.storeF)
-      hipLaunchKernelGGL((k_m2l<R, L>), itemGrid(X.nGroups), dim3(M2LShape<L>::TB), 0, s, X.gBeg,
-                         X.gTarget,
-                         X.gSrcF, X.gMask, X.gCode, (const R2*)F, (const R2*)X.U, Lb, oc.twc,
+    constexpr int G = M2LShape<L>::G;
+    constexpr unsigned TB = M2LShape<L>::TB;
+    if (stored) {
+      // L (aliasing M) is written for every target slot from here on
+      hipLaunchKernelGGL((k_m2l<R, L>), itemGrid(X.nGroups), dim3(TB), 0, s, X.gBeg, X.gTarget,
+                         X.gSrc, X.gMask, X.gCode, (const R2*)F, (const R2*)X.U, Lb, oc.twc,
                          oc.tws, X.nGroups);
-    else
-      hipLaunchKernelGGL((k_m2l_fly<R, L>), itemGrid(X.nGroups), dim3(M2LShape<L>::TB), 0, s,
-                         X.gBeg,
-                         X.gTarget, X.gSrcSlot, X.gMask, X.gCode, (const R2*)ML,
-                         (const R2*)X.U, Lb, oc.twc, oc.tws, X.nGroups);
+    } else if (X.m2lMode == DEFMM_M2L_ON_THE_FLY) {
+      hipLaunchKernelGGL((k_m2l_fly<R, L>), itemGrid(X.nGroups), dim3(TB), 0, s, X.gBeg,
+                         X.gTarget, X.gSrc, X.gMask, X.gCode, (const R2*)ML, (const R2*)X.U, Lb,
+                         oc.twc, oc.tws, X.nGroups);
+    } else {
+      // per batch: its F rows into the workspace, then its groups' M2L
+      // (stream order separates a batch's M2L from the next batch's M2F)
+      const size_t nb = X.hBatchGroupBeg.size() - 1;
+      for (size_t b = 0; b < nb; b++) {
+        const int g0 = X.hBatchGroupBeg[b], g1 = X.hBatchGroupBeg[b + 1];
+        const int r0 = X.hBatchRowBeg[b], r1 = X.hBatchRowBeg[b + 1];
+        hipLaunchKernelGGL((k_m2f<R, L>), itemGrid(r1 - r0), dim3(256), 0, s, X.bRows + r0,
+                           (const R2*)ML, F, oc.twc, oc.tws, r1 - r0);
+        hipLaunchKernelGGL((k_m2l<R, L>), itemGrid(g1 - g0), dim3(TB), 0, s, X.gBeg + g0,
+                           X.gTarget + (size_t)g0 * G, X.gSrc, X.gMask, X.gCode, (const R2*)F,
+                           (const R2*)X.U, Lb, oc.twc, oc.tws, g1 - g0);
+      }
+    }
   }
+  mark(5);
   if (far && X.nM2LZero)
     hipLaunchKernelGGL((k_zero_slots<R, L>), dim3(nblk((int64_t)X.nM2LZero * X.LL, 256)),
                        dim3(256), 0, s, X.m2lZero, X.nM2LZero, Lb);
-  mark(5);
   if (far)
     for (auto& t : X.l2l)
       hipLaunchKernelGGL((k_trans<R, L>), itemGrid(t.n), dim3(128), 0, s, t.out, t.beg, t.in,
@@ -1470,9 +1589,12 @@ static int runTyped(Exec& X, const double2* qin, double2* pout, int parts, hipSt
                        X.ty, X.tz, X.dirs, sig2, (const R2*)Lb, oc, X.kappa, p, X.nl2p);
   mark(7);
   if (near && X.nitems) {
-    // the near field was launched on the side stream right after the gather;
-    // join it, then fold its partial sums into p
-    HIPCK(hipStreamWaitEvent(s, X.evJoin, 0));
+    // near field: per-item partial sums, then folded into p per target leaf.
+    // (Run on a second stream beside M2L it measured no faster: both then
+    // contend for the same CUs.)
+    hipLaunchKernelGGL(k_p2p<R>, itemGrid(X.nitems), dim3(64), 0, s, X.items, X.leafCell,
+                       X.T.pbeg, X.T.nprt, X.tx, X.ty, X.tz, X.sx, X.sy, X.sz, (const R2*)q, X.rb,
+                       X.re, X.kappa, X.partial, X.nitems);
     hipLaunchKernelGGL(k_p2p_reduce<R>, itemGrid(X.nleaves), dim3(64), 0, s, X.itemBeg, X.items,
                        X.leafCell, X.T.pbeg, X.T.nprt, (const double2*)X.partial, p, X.nleaves);
   }
@@ -1481,7 +1603,6 @@ static int runTyped(Exec& X, const double2* qin, double2* pout, int parts, hipSt
                      pout, (int)X.nt);
   mark(9);
   X.evRecorded = true;
-  X.p2pRecorded = near && X.nitems;
   HIPCK(hipGetLastError());
   return OK;
 }
@@ -1564,13 +1685,8 @@ int defmm_plan_create(const double* src_xyz, int64_t n_src, const double* trg_xy
   delete pl;
   if (rc == dfm::OK) {
     if (hipStreamCreate(&h->x.own) != hipSuccess) rc = DEFMM_EHIP;
-    if (rc == dfm::OK && hipStreamCreate(&h->x.side) != hipSuccess) rc = DEFMM_EHIP;
     for (auto& e : h->x.ev)
       if (rc == dfm::OK && hipEventCreate(&e) != hipSuccess) rc = DEFMM_EHIP;
-    for (auto* e : {&h->x.evP2P0, &h->x.evP2P1})
-      if (rc == dfm::OK && hipEventCreate(e) != hipSuccess) rc = DEFMM_EHIP;
-    for (auto* e : {&h->x.evFork, &h->x.evJoin})
-      if (rc == dfm::OK && hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) rc = DEFMM_EHIP;
   }
   if (rc != dfm::OK) {
     delete h;
@@ -1618,7 +1734,8 @@ int defmm_plan_stats(defmm_plan plan, int64_t* out, int n) {
                              s.lf_leaves_p2m, s.hf_m2m, s.lf2hf_m2m, s.lf_m2m, s.hf_l2p,
                              s.lf_l2p, s.hf_l2l, s.hf2lf_l2l, s.lf_l2l, s.nlvl, s.cpt,
                              s.max_stage, s.ncells_t, s.ncells_s, s.p2p_items, s.m2f_slots,
-                             s.m2l_entries, s.m2l_targets};
+                             s.m2l_entries, s.m2l_targets, s.m2l_groups,
+                             s.m2l_group_entries, s.m2l_batches, s.m2l_batch_rows};
   for (int i = 0; i < n && i < DEFMM_NSTATS; i++) out[i] = v[i];
   return DEFMM_OK;
 }
@@ -1631,12 +1748,6 @@ int defmm_stage_times(defmm_plan plan, double* ms, int n) {
   for (int k = 0; k < DEFMM_NSTAGES && k < n; k++) {
     float t = 0.f;
     hipEvent_t a = X.ev[k], b = X.ev[k + 1];
-    if (k == 7) {
-      // the near field ran on the side stream, overlapping stages 4..6
-      if (!X.p2pRecorded) { ms[k] = 0.; continue; }
-      a = X.evP2P0;
-      b = X.evP2P1;
-    }
     if (hipEventElapsedTime(&t, a, b) != hipSuccess) { g_err = "event timing failed"; return DEFMM_EHIP; }
     ms[k] = t;
   }
@@ -1657,17 +1768,39 @@ int defmm_plan_set_precision(defmm_plan plan, int precision) {
   return dfm::uploadPrecision(X);
 }
 
-int defmm_plan_set_m2l(defmm_plan plan, int store_fourier) {
+int defmm_plan_set_m2l(defmm_plan plan, int mode) {
   if (!plan) { g_err = "null plan"; return DEFMM_EARG; }
+  if (mode != DEFMM_M2L_ON_THE_FLY && mode != DEFMM_M2L_STORED && mode != DEFMM_M2L_BATCHED) {
+    g_err = "bad M2L mode";
+    return DEFMM_EARG;
+  }
   Exec& X = plan->x;
-  bool want = store_fourier != 0;
-  if (want == X.storeF) return DEFMM_OK;
-  if (hipSetDevice(X.device) != hipSuccess || hipStreamSynchronize(X.own) != hipSuccess ||
-      hipStreamSynchronize(X.side) != hipSuccess) {
+  if (mode == X.m2lMode) return DEFMM_OK;
+  if (hipSetDevice(X.device) != hipSuccess || hipStreamSynchronize(X.own) != hipSuccess) {
     g_err = "device synchronisation failed";
     return DEFMM_EHIP;
   }
-  X.storeF = want;
+  X.m2lMode = mode;
+  X.evRecorded = false;
+  return dfm::uploadPrecision(X);
+}
+
+int defmm_plan_get_m2l(defmm_plan plan, int* mode) {
+  if (!plan || !mode) { g_err = "null argument"; return DEFMM_EARG; }
+  *mode = plan->x.m2lMode;
+  return DEFMM_OK;
+}
+
+int defmm_plan_set_m2l_batch_rows(defmm_plan plan, int rows) {
+  if (!plan) { g_err = "null plan"; return DEFMM_EARG; }
+  if (rows < 1) { g_err = "batch rows must be positive"; return DEFMM_EARG; }
+  Exec& X = plan->x;
+  X.batchRows = rows;
+  if (X.m2lMode != DEFMM_M2L_BATCHED) return DEFMM_OK;
+  if (hipSetDevice(X.device) != hipSuccess || hipStreamSynchronize(X.own) != hipSuccess) {
+    g_err = "device synchronisation failed";
+    return DEFMM_EHIP;
+  }
   X.evRecorded = false;
   return dfm::uploadPrecision(X);
 }
@@ -1739,7 +1872,6 @@ int defmm_plan_destroy(defmm_plan plan) {
   if (!plan) return DEFMM_OK;
   hipSetDevice(plan->x.device);
   hipStreamSynchronize(plan->x.own);
-  if (plan->x.side) hipStreamSynchronize(plan->x.side);
   delete plan;
   return DEFMM_OK;
 }

@@ -65,6 +65,7 @@ struct Stats {
   int64_t ncells_t = 0, ncells_s = 0;
   int64_t p2p_items = 0, m2f_slots = 0, m2l_entries = 0, m2l_targets = 0;
   int64_t m2l_groups = 0, m2l_group_entries = 0;
+  int64_t m2l_batches = 0, m2l_batch_rows = 0;  // batched M2L (set at upload)
 };
 
 // A per-level translation list (M2M or L2L): for every output slot, the

@@ -87,7 +87,9 @@ int defmm_plan_stats(defmm_plan plan, int64_t* out, int n);
 /*
  * Milliseconds spent in each stage by the last execution (HIP events on the
  * execution stream): writes min(n, DEFMM_NSTAGES) doubles in the order
- * gather, P2M, M2M, M2F, M2L+F2L, L2L, L2P, P2P, scatter.
+ * gather, P2M, M2M, M2F, M2L+F2L, L2L, L2P, P2P, scatter.  All stages run
+ * in this order on the execution stream; the M2L+F2L stage holds the fused
+ * M2L kernel alone (target slots with no M2L are cleared in the L2L stage).
  */
 int defmm_stage_times(defmm_plan plan, double* ms, int n);
 
@@ -117,12 +119,33 @@ int defmm_direct(const double* src_xyz, int64_t n_src, const double* q, const do
 int defmm_plan_set_precision(defmm_plan plan, int precision);
 
 /*
- * M2L variant: store_fourier = 1 keeps the Fourier rows F of every source
- * slot read by M2L in HBM (M2F stage + streamed F); 0 recomputes each row
- * from its M expansion inside M2L (no F store: (2L-1)^3 / L^3 less memory).
- * Results agree to roundoff.  Reallocates the expansion stores.
+ * M2L variants (results agree to roundoff; the switch reallocates the
+ * expansion stores).  defmm_plan_create picks STORED when its stores fit in
+ * 90% of the free device memory, else BATCHED.
+ *  DEFMM_M2L_STORED: the F rows of every source slot read by M2L are kept
+ *    in HBM (one M2F launch, one M2L launch streaming F); M and L share one
+ *    store.  Memory: (2L-1)^3 complex per source slot on top of M.  Fastest.
+ *  DEFMM_M2L_BATCHED: the M2L targets, in (level, direction, cell) order,
+ *    are cut into batches whose source Fourier rows fit a workspace of
+ *    `rows` rows (defmm_plan_set_m2l_batch_rows); per batch one M2F launch
+ *    fills the workspace and one M2L launch reads it while it is still
+ *    cache-resident.  Memory: M and L stores plus the workspace.
+ *  DEFMM_M2L_ON_THE_FLY: M2L recomputes each F row from M inside the
+ *    kernel; no F store.
  */
-int defmm_plan_set_m2l(defmm_plan plan, int store_fourier);
+#define DEFMM_M2L_ON_THE_FLY 0
+#define DEFMM_M2L_STORED 1
+#define DEFMM_M2L_BATCHED 2
+int defmm_plan_set_m2l(defmm_plan plan, int mode);
+/* The plan's current M2L variant. */
+int defmm_plan_get_m2l(defmm_plan plan, int* mode);
+
+/*
+ * Workspace rows of the batched M2L (default 16384).  Raised to the largest
+ * number of source rows a single M2L work group reads if smaller; takes
+ * effect at once when the plan is in batched mode, else at the next switch.
+ */
+int defmm_plan_set_m2l_batch_rows(defmm_plan plan, int rows);
 
 /* Bytes of device memory held by the plan. */
 int defmm_plan_device_bytes(defmm_plan plan, int64_t* bytes);
@@ -141,6 +164,8 @@ enum {
   DEFMM_STAT_HF2LF_L2L, DEFMM_STAT_LF_L2L, DEFMM_STAT_NLVL, DEFMM_STAT_CPT,
   DEFMM_STAT_MAX_STAGE, DEFMM_STAT_NCELLS_T, DEFMM_STAT_NCELLS_S, DEFMM_STAT_P2P_ITEMS,
   DEFMM_STAT_M2F_SLOTS, DEFMM_STAT_M2L_ENTRIES, DEFMM_STAT_M2L_TARGETS,
+  DEFMM_STAT_M2L_GROUPS, DEFMM_STAT_M2L_GROUP_ENTRIES, DEFMM_STAT_M2L_BATCHES,
+  DEFMM_STAT_M2L_BATCH_ROWS,
   DEFMM_NSTATS
 };
 

@@ -35,14 +35,14 @@ def _diff(a, b):
     return np.sqrt((d ** 2).sum() / (np.abs(b) ** 2).sum()), d.max() / np.abs(b).max()
 
 
-def _run_gpu(src, trg, q, L, ncrit, kappa, prec=0, parts=3, store_fourier=None):
+def _run_gpu(src, trg, q, L, ncrit, kappa, prec=0, parts=3, m2l=None, batch_rows=None):
     dm = _gpu()
     A = dm.IBFMM_Mat(precision=prec)
     A.setSourceParticles(src)
     A.setTargetParticles(trg)
     A.prcmpt(L, ncrit, kappa)
-    if store_fourier is not None:
-        A.set_m2l(store_fourier)
+    if m2l is not None:
+        A.set_m2l(m2l, batch_rows=batch_rows)
     p = A.apply(q, parts=parts)
     st = A.stats()
     A.destroy()
@@ -139,23 +139,28 @@ def test_mixed_precision_close_to_fp64(cube, cube_charges, cube_exact):
     assert abs(e2 - 1.13715e-05) / 1.13715e-05 < 0.05
 
 
-@pytest.mark.parametrize("store", [True, False])
-def test_m2l_variants_match_oracle(cube, cube_charges, store):
-    """M2L from stored Fourier rows and from rows recomputed on the fly."""
-    _say(f"case: refined cube L=4 fp64, M2L store_fourier={store}")
+@pytest.mark.parametrize("mode,rows", [(2, None), (2, 64), (1, None), (0, None)])
+def test_m2l_variants_match_oracle(cube, cube_charges, mode, rows):
+    """M2L in cache-sized batches (default, and with a 64-row workspace so
+    that the groups span many batches), from stored Fourier rows, and from
+    rows recomputed on the fly."""
+    _say(f"case: refined cube L=4 fp64, M2L mode={mode} batch_rows={rows}")
     po, _, _, _ = oa.fmm(cube, cube, cube_charges, 4, 32, 1.0, mode=2)
-    pg, _ = _run_gpu(cube, cube, cube_charges, 4, 32, 1.0, parts=2, store_fourier=store)
+    pg, st = _run_gpu(cube, cube, cube_charges, 4, 32, 1.0, parts=2, m2l=mode, batch_rows=rows)
     l2, mx = _diff(pg, po)
-    _say(f"  far field rel L2 {l2:.3e} rel max {mx:.3e}")
+    _say(f"  far field rel L2 {l2:.3e} rel max {mx:.3e} (batches {st['m2l_batches']})")
     assert l2 <= FP64_L2 and mx <= FP64_MAX
+    if mode == 2 and rows is not None:
+        assert st["m2l_batches"] > 10
     from defmm_amd import workloads as wl
     src = wl.sphere(3000, seed=9)
     q = wl.charges(3000, seed=2)
     for L in (3, 7):
         po, _, _, _ = oa.fmm(src, src, q, L, 16, 30.0, mode=2)
-        pg, _ = _run_gpu(src, src, q, L, 16, 30.0, parts=2, store_fourier=store)
+        pg, st = _run_gpu(src, src, q, L, 16, 30.0, parts=2, m2l=mode, batch_rows=rows)
         l2, mx = _diff(pg, po)
-        _say(f"  sphere N=3000 L={L} kappa=30 far field rel L2 {l2:.3e} rel max {mx:.3e}")
+        _say(f"  sphere N=3000 L={L} kappa=30 far field rel L2 {l2:.3e} rel max {mx:.3e} "
+             f"(batches {st['m2l_batches']})")
         assert l2 <= FP64_L2 and mx <= FP64_MAX
 
 
