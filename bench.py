@@ -78,6 +78,8 @@ def main():
                     help="workspace rows of the batched M2L (library default if unset)")
     ap.add_argument("--compare-m2l", action="store_true",
                     help="also time the other M2L variants (FP64)")
+    ap.add_argument("--nrhs", type=int, default=4,
+                    help="right-hand sides of the mixed batched pass (0 to skip)")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="after timing, copy this many GiB device-to-device (a kernel of known "
                          "traffic, to scale rocprof byte counters)")
@@ -186,6 +188,39 @@ def main():
         mixed = {"value": n / (msmx * 1e-3) / 1e6, "ms_per_step": msmx, "rel_l2_err": emx,
                  "rel_linf_err": einfmx, "speedup_vs_fp64": ms64 / msmx, "stage_ms": stmx,
                  "dtype": "f32 storage, f64 phase reduction"}
+    batched = None
+    if not args.no_mixed and args.nrhs > 1:
+        nr = args.nrhs
+        t0 = time.time()
+        A.set_rhs(nr)
+        log(f"width {nr} right-hand sides: {time.time() - t0:.1f} s, device "
+            f"{A.device_bytes() / 1e9:.1f} GB, M2L variant {A.m2l_mode()}")
+        Q = np.stack([q] + [wl.charges(n, seed=100 + r) for r in range(1, nr)])
+        Qd = torch.from_numpy(Q).to(dev)
+        Pd = torch.zeros((nr, n), dtype=torch.complex128, device=dev)
+        for _ in range(args.warmup):
+            A.apply_batch_device(Qd.data_ptr(), Pd.data_ptr(), nr, stream.cuda_stream)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        stages = []
+        e0.record(stream)
+        for _ in range(args.steps):
+            A.apply_batch_device(Qd.data_ptr(), Pd.data_ptr(), nr, stream.cuda_stream)
+            stages.append(A.stage_times())
+        e1.record(stream)
+        torch.cuda.synchronize()
+        msb = e0.elapsed_time(e1) / args.steps
+        p0 = Pd[0].cpu().numpy()[idx]
+        eb = float(np.sqrt((np.abs(p0 - exact) ** 2).sum() / (np.abs(exact) ** 2).sum()))
+        avg = {k: float(np.mean([s[k] for s in stages])) for k in stages[0]}
+        log(f"[mixed x{nr}] {msb:.2f} ms/pass, {msb / nr:.2f} ms per right-hand side, rel L2 "
+            f"(first) {eb:.3e}, stages {avg}")
+        batched = {"value": n * nr / (msb * 1e-3) / 1e6, "nrhs": nr, "ms_per_step": msb,
+                   "rel_l2_err_first_rhs": eb, "speedup_vs_fp64": ms64 * nr / msb,
+                   "stage_ms": avg, "m2l_variant": ["on the fly", "stored", "batched"][A.m2l_mode()],
+                   "device_gb": A.device_bytes() / 1e9,
+                   "dtype": "f32 storage, f64 phase reduction"}
+        del Qd, Pd
     A.destroy()
     if args.copy_gib > 0:
         nb = int(args.copy_gib * (1 << 30)) // 16
@@ -219,6 +254,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "mixed": mixed,
+        "mixed_batched": batched,
         "m2l_variants_fp64": alt,
         "plan": {k: st[k] for k in ("p2p_pairs", "hf_m2l", "src_slots", "trg_slots",
                                       "m2f_slots", "m2l_entries", "m2l_group_entries",

@@ -67,6 +67,9 @@ def lib():
     L.defmm_execute.argtypes = [P, D, D]
     L.defmm_execute_device.argtypes = [P, P, P, P]
     L.defmm_execute_parts.argtypes = [P, D, D, ctypes.c_int]
+    L.defmm_execute_batch.argtypes = [P, D, D, ctypes.c_int]
+    L.defmm_execute_batch_device.argtypes = [P, P, P, ctypes.c_int, P]
+    L.defmm_plan_set_rhs.argtypes = [P, ctypes.c_int]
     L.defmm_plan_stats.argtypes = [P, I64, ctypes.c_int]
     L.defmm_stage_times.argtypes = [P, D, ctypes.c_int]
     L.defmm_plan_device_bytes.argtypes = [P, I64]
@@ -209,6 +212,30 @@ class IBFMM_Mat:
         self._need_plan()
         _check(lib().defmm_execute_device(self._plan, ctypes.c_void_p(q_ptr),
                                           ctypes.c_void_p(p_ptr), ctypes.c_void_p(stream or 0)))
+
+    def set_rhs(self, nrhs):
+        """Width of the expansion stores in right-hand sides (reallocates)."""
+        self._need_plan()
+        _check(lib().defmm_plan_set_rhs(self._plan, int(nrhs)))
+
+    def apply_batch(self, Q):
+        """Evaluate P = A Q for several right-hand sides: Q (nrhs, n_src)
+        complex -> P (nrhs, n_trg)."""
+        self._need_plan()
+        Q = np.ascontiguousarray(np.asarray(Q, dtype=np.complex128))
+        if Q.ndim != 2 or Q.shape[1] != self.n_src:
+            raise DefmmError(f"expected (nrhs, {self.n_src}) charges, got {Q.shape}")
+        P = np.zeros((Q.shape[0], self.n_trg), dtype=np.complex128)
+        _check(lib().defmm_execute_batch(self._plan, _dptr(Q.view(np.float64)),
+                                         _dptr(P.view(np.float64)), Q.shape[0]))
+        return P
+
+    def apply_batch_device(self, q_ptr, p_ptr, nrhs, stream=None):
+        """Several right-hand sides with device pointers (nrhs x n complex128)."""
+        self._need_plan()
+        _check(lib().defmm_execute_batch_device(self._plan, ctypes.c_void_p(q_ptr),
+                                                ctypes.c_void_p(p_ptr), int(nrhs),
+                                                ctypes.c_void_p(stream or 0)))
 
     def set_precision(self, precision):
         """Switch an existing plan to FP64 or MIXED (lists are kept)."""

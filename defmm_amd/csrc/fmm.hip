@@ -389,6 +389,23 @@ struct M2LShape {
   static constexpr int TB = ((PP + NW - 1) / NW + 63) / 64 * 64;
   static constexpr int G = kM2LGroupMax / 2;
 };
+// With NR right-hand sides per launch a thread holds G x NW x NR
+// accumulators: fewest frequencies per thread (blocks of up to 1024 threads).
+template <int L, int NR>
+struct M2LShapeN {
+  static constexpr int PP = M2LShape<L>::PP;
+  static constexpr int NW = NR == 1 ? M2LShape<L>::NW : (PP + 1023) / 1024;
+  static constexpr int TB = NR == 1 ? M2LShape<L>::TB : ((PP + NW - 1) / NW + 63) / 64 * 64;
+  static constexpr int G = M2LShape<L>::G;
+};
+// Right-hand sides per M2L launch: at most 16 (double) or 32 (float)
+// complex accumulators per thread.
+template <typename R, int L>
+constexpr int m2lMaxNR() {
+  constexpr int NW = (M2LShape<L>::PP + 1023) / 1024;
+  constexpr int cap = (sizeof(R) == 8 ? 16 : 32) / (M2LShape<L>::G * NW);
+  return cap >= 8 ? 8 : cap >= 4 ? 4 : cap >= 2 ? 2 : 1;
+}
 
 template <int P>
 __device__ __forceinline__ int flipIndex(int z, int w0, int w1, int w2) {
@@ -402,20 +419,18 @@ __device__ __forceinline__ size_t tableRow(int code) {
   return (size_t)(code >> 3) * PP;
 }
 
-template <typename R, int L>
-__global__ __launch_bounds__(M2LShape<L>::TB) void k_m2l(const int* __restrict__ groupBeg,
-                                             const int* __restrict__ groupTarget,
-                                             const int* __restrict__ gSrcF,
-                                             const int* __restrict__ gMask,
-                                             const int* __restrict__ gCode,
-                                             const typename C2T<R>::T* __restrict__ F,
-                                             const typename C2T<R>::T* __restrict__ U,
-                                             typename C2T<R>::T* __restrict__ Lout,
-                                             const R* __restrict__ twc,
-                                             const R* __restrict__ tws, int nItems) {
+// NR right-hand sides per launch: right-hand side r reads F + r * fStride
+// and writes Lout + r * lStride; each table value is loaded once for all.
+template <typename R, int L, int NR>
+__global__ __launch_bounds__((M2LShapeN<L, NR>::TB)) void k_m2l(
+    const int* __restrict__ groupBeg, const int* __restrict__ groupTarget,
+    const int* __restrict__ gSrcF, const int* __restrict__ gMask, const int* __restrict__ gCode,
+    const typename C2T<R>::T* __restrict__ F, size_t fStride,
+    const typename C2T<R>::T* __restrict__ U, typename C2T<R>::T* __restrict__ Lout,
+    size_t lStride, const R* __restrict__ twc, const R* __restrict__ tws, int nItems) {
   typedef typename C2T<R>::T R2;
   constexpr int P = 2 * L - 1, LL = L * L * L, PP = P * P * P;
-  constexpr int NW = M2LShape<L>::NW, TB = M2LShape<L>::TB, G = M2LShape<L>::G;
+  constexpr int NW = M2LShapeN<L, NR>::NW, TB = M2LShapeN<L, NR>::TB, G = M2LShapeN<L, NR>::G;
   __shared__ R2 J[PP], T1[L * P * P], T2[L * L * P];
   __shared__ R wc[P], ws[P];
   const int tid = threadIdx.x;
@@ -442,11 +457,13 @@ __global__ __launch_bounds__(M2LShape<L>::TB) void k_m2l(const int* __restrict__
     w1[m] = (w / P) % P;
     w2[m] = w % P;
   }
-  R2 acc[G][NW];
+  R2 acc[G][NW][NR];
 #pragma unroll
   for (int g = 0; g < G; g++)
 #pragma unroll
-    for (int m = 0; m < NW; m++) acc[g][m] = c2<R>(0, 0);
+    for (int m = 0; m < NW; m++)
+#pragma unroll
+      for (int r = 0; r < NR; r++) acc[g][m][r] = c2<R>(0, 0);
   const int e0 = groupBeg[grp], e1 = groupBeg[grp + 1];
   for (int e = e0; e < e1; e++) {
     const R2* Fr = F + (size_t)gSrcF[e] * PP;
@@ -455,9 +472,11 @@ __global__ __launch_bounds__(M2LShape<L>::TB) void k_m2l(const int* __restrict__
 #pragma unroll
     for (int g = 0; g < G; g++) code[g] = gCode[(size_t)e * G + g];
     // issue all of the entry's loads before the first use
-    R2 f[NW], u[G][NW];
+    R2 f[NW][NR], u[G][NW];
 #pragma unroll
-    for (int m = 0; m < NW; m++) f[m] = Fr[wl[m]];
+    for (int m = 0; m < NW; m++)
+#pragma unroll
+      for (int r = 0; r < NR; r++) f[m][r] = Fr[r * fStride + wl[m]];
 #pragma unroll
     for (int g = 0; g < G; g++) {
       const R2* Ur = U + tableRow<PP>(code[g]);
@@ -474,10 +493,12 @@ __global__ __launch_bounds__(M2LShape<L>::TB) void k_m2l(const int* __restrict__
     for (int g = 0; g < G; g++) {
       if (!((mask >> g) & 1)) continue;
 #pragma unroll
-      for (int m = 0; m < NW; m++) {
-        acc[g][m].x += u[g][m].x * f[m].x - u[g][m].y * f[m].y;
-        acc[g][m].y += u[g][m].x * f[m].y + u[g][m].y * f[m].x;
-      }
+      for (int m = 0; m < NW; m++)
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+          acc[g][m][r].x += u[g][m].x * f[m][r].x - u[g][m].y * f[m][r].y;
+          acc[g][m][r].y += u[g][m].x * f[m][r].y + u[g][m].y * f[m][r].x;
+        }
     }
   }
   const R inv = (R)(1.0 / (double)PP);
@@ -485,12 +506,14 @@ __global__ __launch_bounds__(M2LShape<L>::TB) void k_m2l(const int* __restrict__
   for (int g = 0; g < G; g++) {
     const int t = groupTarget[(size_t)grp * G + g];
     if (t < 0) continue;
-    R2* Lr = Lout + (size_t)t * LL;
+#pragma unroll
+    for (int rr = 0; rr < NR; rr++) {
+    R2* Lr = Lout + rr * lStride + (size_t)t * LL;
     __syncthreads();
 #pragma unroll
     for (int m = 0; m < NW; m++) {
       const int w = tid + TB * m;
-      if (w < PP) J[w] = acc[g][m];
+      if (w < PP) J[w] = acc[g][m][rr];
     }
     __syncthreads();
     // axis 0: T1[(k0*P + w1)*P + w2], k0 < L
@@ -533,6 +556,7 @@ __global__ __launch_bounds__(M2LShape<L>::TB) void k_m2l(const int* __restrict__
         s.y += v.y * wc[m] + v.x * ws[m];
       }
       Lr[x] = c2<R>(s.x * inv, s.y * inv);
+    }
     }
   }
 }
@@ -819,31 +843,29 @@ struct P2PItem {
   int cnt;   // number of sources
 };
 
+// Kernel value G = exp(i kappa r) / (4 pi r) between a target and a source
+// (P2P.hpp:61-75), 0 when r^2 < 1e-16.
 template <typename R>
-__device__ __forceinline__ void p2pTerm(double xt, double yt, double zt, double xs, double ys,
-                                        double zs, typename C2T<R>::T qs, double kappa,
-                                        R& pr, R& pi);
+__device__ __forceinline__ void p2pKernel(double xt, double yt, double zt, double xs, double ys,
+                                          double zs, double kappa, R& co, R& si);
 template <>
-__device__ __forceinline__ void p2pTerm<double>(double xt, double yt, double zt, double xs,
-                                                double ys, double zs, double2 qs, double kappa,
-                                                double& pr, double& pi) {
+__device__ __forceinline__ void p2pKernel<double>(double xt, double yt, double zt, double xs,
+                                                  double ys, double zs, double kappa,
+                                                  double& co, double& si) {
   const double x = xt - xs, y = yt - ys, z = zt - zs;
   const double R2v = x * x + y * y + z * z;
   double K = rsqrt(R2v);
   if (R2v < 1.e-16) K = 0.;
   const double r = K * R2v * kappa;
   K *= 0.079577471545947667884;  // 1/(4 pi)
-  double si, co;
   sincos(r, &si, &co);
   co *= K;
   si *= K;
-  pr += co * qs.x - si * qs.y;
-  pi += co * qs.y + si * qs.x;
 }
 template <>
-__device__ __forceinline__ void p2pTerm<float>(double xt, double yt, double zt, double xs,
-                                               double ys, double zs, float2 qs, double kappa,
-                                               float& pr, float& pi) {
+__device__ __forceinline__ void p2pKernel<float>(double xt, double yt, double zt, double xs,
+                                                 double ys, double zs, double kappa, float& co,
+                                                 float& si) {
   const double x = xt - xs, y = yt - ys, z = zt - zs;
   const double R2v = x * x + y * y + z * z;
   // r to ~1e-14 relative: float reciprocal square root, one Newton step in double
@@ -851,15 +873,14 @@ __device__ __forceinline__ void p2pTerm<float>(double xt, double yt, double zt, 
   const double r0 = R2v * (double)y0;
   const double r = fma(0.5 * (double)y0, fma(-r0, r0, R2v), r0);
   const float K = y0 * 0.079577471545947667884f;
-  float co, si;
   expi(kappa * r, co, si);
   co *= K;
   si *= K;
-  pr += co * qs.x - si * qs.y;
-  pi += co * qs.y + si * qs.x;
 }
 
-template <typename R>
+// NR right-hand sides: charges q + r * qStride; the partial sums of item k,
+// right-hand side r, lane t at partial[(k * NR + r) * 64 + t].
+template <typename R, int NR>
 __global__ __launch_bounds__(64) void k_p2p(const P2PItem* __restrict__ items,
                                             const int* __restrict__ leafCell,
                                             const int* __restrict__ tpbeg,
@@ -871,13 +892,14 @@ __global__ __launch_bounds__(64) void k_p2p(const P2PItem* __restrict__ items,
                                             const double* __restrict__ sy,
                                             const double* __restrict__ sz,
                                             const typename C2T<R>::T* __restrict__ q,
+                                            size_t qStride,
                                             const int* __restrict__ rb, const int* __restrict__ re,
                                             double kappa, double2* __restrict__ partial,
                                             int nItems) {
   typedef typename C2T<R>::T R2;
   __shared__ double lx[64], ly[64], lz[64];
-  __shared__ R2 lq[64];
-  __shared__ double2 red[64];
+  __shared__ R2 lq[NR][64];
+  __shared__ double2 red[NR][64];
   const int item = itemIndex();
   if (item >= nItems) return;
   const P2PItem it = items[item];
@@ -889,7 +911,9 @@ __global__ __launch_bounds__(64) void k_p2p(const P2PItem* __restrict__ items,
   const int ti = lane % nt, g = lane / nt;
   const bool active = g < G;
   const double xt = tx[tb + ti], yt = ty[tb + ti], zt = tz[tb + ti];
-  double accr = 0., acci = 0.;
+  double accr[NR], acci[NR];
+#pragma unroll
+  for (int k = 0; k < NR; k++) accr[k] = acci[k] = 0.;
   int r = it.r, off = it.off;
   for (int pos = 0; pos < it.cnt; pos += 64) {
     const int tileN = min(64, it.cnt - pos);
@@ -906,7 +930,8 @@ __global__ __launch_bounds__(64) void k_p2p(const P2PItem* __restrict__ items,
       lx[lane] = sx[j];
       ly[lane] = sy[j];
       lz[lane] = sz[j];
-      lq[lane] = q[j];
+#pragma unroll
+      for (int k = 0; k < NR; k++) lq[k][lane] = q[k * qStride + j];
     }
     {
       int adv = tileN;
@@ -924,27 +949,47 @@ __global__ __launch_bounds__(64) void k_p2p(const P2PItem* __restrict__ items,
     }
     __syncthreads();
     if (active) {
-      R pr = 0, pi = 0;
-      for (int j = g; j < tileN; j += G) p2pTerm<R>(xt, yt, zt, lx[j], ly[j], lz[j], lq[j], kappa, pr, pi);
-      accr += (double)pr;
-      acci += (double)pi;
+      R pr[NR], pi[NR];
+#pragma unroll
+      for (int k = 0; k < NR; k++) pr[k] = pi[k] = 0;
+      for (int j = g; j < tileN; j += G) {
+        R co, si;
+        p2pKernel<R>(xt, yt, zt, lx[j], ly[j], lz[j], kappa, co, si);
+#pragma unroll
+        for (int k = 0; k < NR; k++) {
+          const R2 qs = lq[k][j];
+          pr[k] += co * qs.x - si * qs.y;
+          pi[k] += co * qs.y + si * qs.x;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < NR; k++) {
+        accr[k] += (double)pr[k];
+        acci[k] += (double)pi[k];
+      }
     }
   }
-  red[lane] = make_double2(accr, acci);
+#pragma unroll
+  for (int k = 0; k < NR; k++) red[k][lane] = make_double2(accr[k], acci[k]);
   __syncthreads();
   if (lane < nt) {
-    double2 s = red[lane];
-    for (int gg = 1; gg < G; gg++) {
-      s.x += red[gg * nt + lane].x;
-      s.y += red[gg * nt + lane].y;
+#pragma unroll
+    for (int k = 0; k < NR; k++) {
+      double2 s = red[k][lane];
+      for (int gg = 1; gg < G; gg++) {
+        s.x += red[k][gg * nt + lane].x;
+        s.y += red[k][gg * nt + lane].y;
+      }
+      partial[((size_t)item * NR + k) * 64 + lane] = s;
     }
-    partial[(size_t)item * 64 + lane] = s;
   }
 }
 
 // Sum the partial results of a leaf's items into p (items of a leaf are
 // contiguous, ordered by target chunk).
-template <typename R>
+// NR right-hand sides (partial layout of k_p2p<R, NR>); right-hand side r
+// adds into p + r * pStride.
+template <typename R, int NR>
 __global__ __launch_bounds__(64) void k_p2p_reduce(const int* __restrict__ itemBeg,
                                                    const P2PItem* __restrict__ items,
                                                    const int* __restrict__ leafCell,
@@ -952,7 +997,7 @@ __global__ __launch_bounds__(64) void k_p2p_reduce(const int* __restrict__ itemB
                                                    const int* __restrict__ tnprt,
                                                    const double2* __restrict__ partial,
                                                    typename C2T<R>::T* __restrict__ p,
-                                                   int nItems) {
+                                                   size_t pStride, int nItems) {
   typedef typename C2T<R>::T R2;
   const int leaf = itemIndex();
   if (leaf >= nItems) return;
@@ -961,18 +1006,26 @@ __global__ __launch_bounds__(64) void k_p2p_reduce(const int* __restrict__ itemB
   const int ib = itemBeg[leaf], ie = itemBeg[leaf + 1];
   for (int t0 = 0; t0 < n; t0 += 64) {
     const int i = t0 + threadIdx.x;
-    double sr = 0., si = 0.;
+    double sr[NR], si[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) sr[r] = si[r] = 0.;
     for (int k = ib; k < ie; k++) {
       if (items[k].t0 != t0) continue;
-      const double2 v = partial[(size_t)k * 64 + threadIdx.x];
-      sr += v.x;
-      si += v.y;
+#pragma unroll
+      for (int r = 0; r < NR; r++) {
+        const double2 v = partial[((size_t)k * NR + r) * 64 + threadIdx.x];
+        sr[r] += v.x;
+        si[r] += v.y;
+      }
     }
     if (i < n) {
-      R2 o = p[pb + i];
-      o.x = (R)((double)o.x + sr);
-      o.y = (R)((double)o.y + si);
-      p[pb + i] = o;
+#pragma unroll
+      for (int r = 0; r < NR; r++) {
+        R2 o = p[r * pStride + pb + i];
+        o.x = (R)((double)o.x + sr[r]);
+        o.y = (R)((double)o.y + si[r]);
+        p[r * pStride + pb + i] = o;
+      }
     }
   }
 }
@@ -1093,7 +1146,10 @@ struct Exec {
   double2* partial = nullptr;
   void *q = nullptr, *p = nullptr, *ML = nullptr, *F = nullptr;
   void* Lb = nullptr;        // local expansions (aliases ML when F is stored)
+  size_t mStride = 0, lStride = 0, fStride = 0;  // elements between right-hand sides
   int m2lMode = DEFMM_M2L_STORED;
+  bool m2lAuto = true;       // mode chosen by free memory (until set explicitly)
+  int nr = 1;                // right-hand sides the expansion stores hold
   int batchRows = 16384;     // requested workspace rows of the batched M2L
   int wsRows = 0;            // rows of the workspace in use
   int* bRows = nullptr;      // batched M2L: source slots of the batches' rows
@@ -1291,17 +1347,23 @@ static int uploadTyped(Exec& X) {
     CKR(X.upv(d, c));
     X.consts = d;
   }
+  // right-hand side r of every store at base + r * stride
+  const size_t nr = (size_t)X.nr;
   if (X.m2lMode == DEFMM_M2L_STORED) {
     // M is dead after M2F and every target slot's L is written by M2L or
     // zeroed: one buffer holds M, then L
-    CKR(X.alloc(X.ML, X.nMLslots * X.LL * sizeof(R2)));
-    CKR(X.alloc(X.F, (size_t)std::max(1, X.nm2f) * X.PP * sizeof(R2)));
+    X.mStride = X.lStride = X.nMLslots * X.LL;
+    X.fStride = (size_t)std::max(1, X.nm2f) * X.PP;
+    CKR(X.alloc(X.ML, nr * X.mStride * sizeof(R2)));
+    CKR(X.alloc(X.F, nr * X.fStride * sizeof(R2)));
     X.Lb = X.ML;
     CKR(X.upv(X.gSrc, X.hGSrcF));
   } else {
     // M2L reads M while writing L: two buffers
-    CKR(X.alloc(X.ML, (size_t)std::max(1, X.nSslots) * X.LL * sizeof(R2)));
-    CKR(X.alloc(X.Lb, (size_t)std::max(1, X.nTslots) * X.LL * sizeof(R2)));
+    X.mStride = (size_t)std::max(1, X.nSslots) * X.LL;
+    X.lStride = (size_t)std::max(1, X.nTslots) * X.LL;
+    CKR(X.alloc(X.ML, nr * X.mStride * sizeof(R2)));
+    CKR(X.alloc(X.Lb, nr * X.lStride * sizeof(R2)));
     X.F = nullptr;
     if (X.m2lMode == DEFMM_M2L_ON_THE_FLY) {
       std::vector<int> ss(X.hGSrcF.size());
@@ -1312,12 +1374,42 @@ static int uploadTyped(Exec& X) {
       CKR(buildBatches(X, local, rows));
       CKR(X.upv(X.gSrc, local));
       CKR(X.upv(X.bRows, rows));
-      CKR(X.alloc(X.F, (size_t)X.wsRows * X.PP * sizeof(R2)));
+      X.fStride = (size_t)X.wsRows * X.PP;
+      CKR(X.alloc(X.F, nr * X.fStride * sizeof(R2)));
     }
   }
-  CKR(X.alloc(X.q, (size_t)X.ns * sizeof(R2)));
-  CKR(X.alloc(X.p, (size_t)X.nt * sizeof(R2)));
+  CKR(X.alloc(X.q, nr * (size_t)X.ns * sizeof(R2)));
+  CKR(X.alloc(X.p, nr * (size_t)X.nt * sizeof(R2)));
+  {
+    void* d;
+    CKR(X.alloc(d, nr * (size_t)std::max(1, X.nitems) * 64 * sizeof(double2)));
+    X.partial = (double2*)d;
+    CKR(X.alloc(d, nr * (size_t)X.ns * sizeof(double2)));
+    X.qio = (double2*)d;
+    CKR(X.alloc(d, nr * (size_t)X.nt * sizeof(double2)));
+    X.pio = (double2*)d;
+  }
   X.typedPhase = false;
+  return OK;
+}
+
+// Default M2L variant: stored F rows (one M2F and one M2L launch, the
+// faster variant) when they fit in 90% of the free device memory with the
+// shared M/L store for all right-hand sides, else cache-sized batches
+// (separate M and L stores).  Call with the typed stores released.
+static int chooseM2LMode(Exec& X) {
+  if (!X.m2lAuto) return OK;
+  size_t freeB = 0, totalB = 0;
+  if (hipMemGetInfo(&freeB, &totalB) != hipSuccess) {
+    g_err = "hipMemGetInfo failed";
+    return ERR_HIP;
+  }
+  const size_t r2 = X.prec == DEFMM_FP64 ? 16 : 8;
+  const size_t storedB =
+      (size_t)X.nr * (((size_t)X.nm2f * X.PP + X.nMLslots * X.LL) * r2 + ((size_t)X.ns + X.nt) * (r2 + 16) +
+                      (size_t)X.nitems * 64 * 16) +
+      (size_t)X.hGSrcF.size() * sizeof(int);
+  X.m2lMode = (double)storedB < 0.9 * (double)freeB ? DEFMM_M2L_STORED : DEFMM_M2L_BATCHED;
   return OK;
 }
 
@@ -1465,43 +1557,63 @@ static int upload(Exec& X, Plan& pl) {
     CKR(X.upv(X.leafCell, pl.p2pLeaf));
     CKR(X.upv(X.rb, pl.p2pSrcB));
     CKR(X.upv(X.re, pl.p2pSrcE));
-    void* pp;
-    CKR(X.alloc(pp, (size_t)std::max(1, X.nitems) * 64 * sizeof(double2)));
-    X.partial = (double2*)pp;
-  }
-  {
-    void* d;
-    CKR(X.alloc(d, (size_t)X.ns * sizeof(double2)));
-    X.qio = (double2*)d;
-    CKR(X.alloc(d, (size_t)X.nt * sizeof(double2)));
-    X.pio = (double2*)d;
   }
   X.hSig.swap(pl.sig);
   X.hSig2.swap(pl.sig2);
   X.hTab.swap(pl.m2lTab);
   for (int s = 0; s < 2; s++) { X.hA[s] = pl.A[s]; X.hB[s] = pl.B[s]; }
   X.nMLslots = (size_t)std::max(pl.S.nslots, pl.T.nslots);
-  // default M2L variant: stored F rows (one M2F and one M2L launch, the
-  // faster variant) when they fit in 90% of the free device memory with the
-  // shared M/L store, else cache-sized batches (separate M and L stores)
-  {
-    size_t freeB = 0, totalB = 0;
-    if (hipMemGetInfo(&freeB, &totalB) != hipSuccess) {
-      g_err = "hipMemGetInfo failed";
-      return ERR_HIP;
-    }
-    const size_t r2 = X.prec == DEFMM_FP64 ? 16 : 8;
-    const size_t storedB = ((size_t)X.nm2f * X.PP + X.nMLslots * X.LL) * r2 +
-                           (size_t)X.hGSrcF.size() * sizeof(int) + ((size_t)X.ns + X.nt) * r2;
-    X.m2lMode = (double)storedB < 0.9 * (double)freeB ? DEFMM_M2L_STORED : DEFMM_M2L_BATCHED;
-  }
+  CKR(chooseM2LMode(X));
   return uploadPrecision(X);
 }
 
 static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
+// Launchers of the NR-wide kernels: the widest NR in {8, 4, 2, 1} that is
+// <= n and allowed, for right-hand sides r0 .. r0 + NR - 1.
+template <typename R, int L, int NR>
+static void launchM2L(Exec& X, hipStream_t s, int g0, int ng, const typename C2T<R>::T* F,
+                      typename C2T<R>::T* Lb, const OpConsts<R>& oc) {
+  typedef typename C2T<R>::T R2;
+  constexpr int G = M2LShapeN<L, NR>::G;
+  hipLaunchKernelGGL((k_m2l<R, L, NR>), itemGrid(ng), dim3(M2LShapeN<L, NR>::TB), 0, s,
+                     X.gBeg + g0, X.gTarget + (size_t)g0 * G, X.gSrc, X.gMask, X.gCode, F,
+                     X.fStride, (const R2*)X.U, Lb, X.lStride, oc.twc, oc.tws, ng);
+}
+
 template <typename R, int L>
-static int runTyped(Exec& X, const double2* qin, double2* pout, int parts, hipStream_t s) {
+static int m2lChunk(Exec& X, hipStream_t s, int g0, int ng, int r0, int n, const OpConsts<R>& oc) {
+  typedef typename C2T<R>::T R2;
+  const R2* F = (const R2*)X.F + (size_t)r0 * X.fStride;
+  R2* Lb = (R2*)X.Lb + (size_t)r0 * X.lStride;
+  constexpr int mx = m2lMaxNR<R, L>();
+  if (mx >= 8 && n >= 8) { launchM2L<R, L, (mx >= 8 ? 8 : 1)>(X, s, g0, ng, F, Lb, oc); return 8; }
+  if (mx >= 4 && n >= 4) { launchM2L<R, L, (mx >= 4 ? 4 : 1)>(X, s, g0, ng, F, Lb, oc); return 4; }
+  if (mx >= 2 && n >= 2) { launchM2L<R, L, (mx >= 2 ? 2 : 1)>(X, s, g0, ng, F, Lb, oc); return 2; }
+  launchM2L<R, L, 1>(X, s, g0, ng, F, Lb, oc);
+  return 1;
+}
+
+template <typename R, int NR>
+static void launchP2P(Exec& X, hipStream_t s, int r0) {
+  typedef typename C2T<R>::T R2;
+  const R2* q = (const R2*)X.q + (size_t)r0 * X.ns;
+  R2* p = (R2*)X.p + (size_t)r0 * X.nt;
+  hipLaunchKernelGGL((k_p2p<R, NR>), itemGrid(X.nitems), dim3(64), 0, s, X.items, X.leafCell,
+                     X.T.pbeg, X.T.nprt, X.tx, X.ty, X.tz, X.sx, X.sy, X.sz, q, (size_t)X.ns, X.rb,
+                     X.re, X.kappa, X.partial, X.nitems);
+  hipLaunchKernelGGL((k_p2p_reduce<R, NR>), itemGrid(X.nleaves), dim3(64), 0, s, X.itemBeg,
+                     X.items, X.leafCell, X.T.pbeg, X.T.nprt, (const double2*)X.partial, p,
+                     (size_t)X.nt, X.nleaves);
+}
+
+// One evaluation pass for nrhs <= X.nr right-hand sides: qin / pout hold
+// nrhs x n complex128 values, right-hand side after right-hand side, in
+// input order.  Stages as in the reference's gemv; M2L and P2P process up to
+// 8 right-hand sides per launch, the other stages one per launch.
+template <typename R, int L>
+static int runTyped(Exec& X, const double2* qin, double2* pout, int parts, int nrhs,
+                    hipStream_t s) {
   typedef typename C2T<R>::T R2;
   constexpr int P = 2 * L - 1;
   const R* cst = (const R*)X.consts;
@@ -1514,93 +1626,108 @@ static int runTyped(Exec& X, const double2* qin, double2* pout, int parts, hipSt
   oc.B[1] = oc.B[0] + L * L;
   oc.twc = oc.B[1] + L * L;
   oc.tws = oc.twc + P;
-  R2* q = (R2*)X.q;
-  R2* p = (R2*)X.p;
-  R2* ML = (R2*)X.ML;
-  R2* Lb = (R2*)X.Lb;
-  R2* F = (R2*)X.F;
-  const R2* sig = (const R2*)X.sig;
-  const R2* sig2 = (const R2*)X.sig2;
-  auto mark = [&](int k) { hipEventRecord(X.ev[k], s); };
-  mark(0);
-  hipLaunchKernelGGL(k_zero<R2>, dim3(nblk(X.nt, 256)), dim3(256), 0, s, p, (int64_t)X.nt);
-  hipLaunchKernelGGL(k_gather<R>, dim3(nblk(X.ns, 256)), dim3(256), 0, s, qin, X.sperm, q, (int)X.ns);
-  mark(1);
-  const bool far = (parts & 2) != 0, near = (parts & 1) != 0;
-
-  if (far && X.np2m)
-    hipLaunchKernelGGL((k_p2m<R, L>), itemGrid(X.np2m), dim3(128), 0, s, X.p2m, X.S, X.lv, X.sx,
-                       X.sy, X.sz, (const R2*)q, X.dirs, sig2, oc, X.kappa, ML, X.np2m);
-  mark(2);
-  if (far)
-    for (auto& t : X.m2m)
-      hipLaunchKernelGGL((k_trans<R, L>), itemGrid(t.n), dim3(128), 0, s, t.out, t.beg, t.in,
-                         t.sgn, t.sigIdx, t.edv, (const R2*)ML, ML, sig, sig2, oc.A[0], oc.A[1], 0,
-                         t.n);
-  mark(3);
-  const bool stored = X.m2lMode == DEFMM_M2L_STORED;
-  if (far && X.nm2f && stored)
-    hipLaunchKernelGGL((k_m2f<R, L>), itemGrid(X.nm2f), dim3(256), 0, s, X.m2fSlot, (const R2*)ML,
-                       F, oc.twc, oc.tws, X.nm2f);
-  mark(4);
+  if (nrhs < 1 || nrhs > X.nr) {
+    g_err = "right-hand sides exceed the plan's width";
+    return ERR_ARG;
+  }
   if (X.m2lG != M2LShape<L>::G) {
     g_err = "internal: M2L group size mismatch";
     return ERR_STATE;
   }
+  const size_t ns = (size_t)X.ns, nt = (size_t)X.nt;
+  auto qr = [&](int r) { return (R2*)X.q + r * ns; };
+  auto pr = [&](int r) { return (R2*)X.p + r * nt; };
+  auto Mr = [&](int r) { return (R2*)X.ML + r * X.mStride; };
+  auto Lr = [&](int r) { return (R2*)X.Lb + r * X.lStride; };
+  auto Fr = [&](int r) { return (R2*)X.F + r * X.fStride; };
+  const R2* sig = (const R2*)X.sig;
+  const R2* sig2 = (const R2*)X.sig2;
+  auto mark = [&](int k) { hipEventRecord(X.ev[k], s); };
+  mark(0);
+  hipLaunchKernelGGL(k_zero<R2>, dim3(nblk((int64_t)(nrhs * nt), 256)), dim3(256), 0, s,
+                     (R2*)X.p, (int64_t)(nrhs * nt));
+  for (int r = 0; r < nrhs; r++)
+    hipLaunchKernelGGL(k_gather<R>, dim3(nblk(X.ns, 256)), dim3(256), 0, s, qin + r * ns,
+                       X.sperm, qr(r), (int)X.ns);
+  mark(1);
+  const bool far = (parts & 2) != 0, near = (parts & 1) != 0;
+
+  if (far && X.np2m)
+    for (int r = 0; r < nrhs; r++)
+      hipLaunchKernelGGL((k_p2m<R, L>), itemGrid(X.np2m), dim3(128), 0, s, X.p2m, X.S, X.lv,
+                         X.sx, X.sy, X.sz, (const R2*)qr(r), X.dirs, sig2, oc, X.kappa, Mr(r),
+                         X.np2m);
+  mark(2);
+  if (far)
+    for (int r = 0; r < nrhs; r++)
+      for (auto& t : X.m2m)
+        hipLaunchKernelGGL((k_trans<R, L>), itemGrid(t.n), dim3(128), 0, s, t.out, t.beg, t.in,
+                           t.sgn, t.sigIdx, t.edv, (const R2*)Mr(r), Mr(r), sig, sig2, oc.A[0],
+                           oc.A[1], 0, t.n);
+  mark(3);
+  const bool stored = X.m2lMode == DEFMM_M2L_STORED;
+  if (far && X.nm2f && stored)
+    for (int r = 0; r < nrhs; r++)
+      hipLaunchKernelGGL((k_m2f<R, L>), itemGrid(X.nm2f), dim3(256), 0, s, X.m2fSlot,
+                         (const R2*)Mr(r), Fr(r), oc.twc, oc.tws, X.nm2f);
+  mark(4);
   if (far && X.nGroups) {
-    constexpr int G = M2LShape<L>::G;
-    constexpr unsigned TB = M2LShape<L>::TB;
     if (stored) {
       // L (aliasing M) is written for every target slot from here on
-      hipLaunchKernelGGL((k_m2l<R, L>), itemGrid(X.nGroups), dim3(TB), 0, s, X.gBeg, X.gTarget,
-                         X.gSrc, X.gMask, X.gCode, (const R2*)F, (const R2*)X.U, Lb, oc.twc,
-                         oc.tws, X.nGroups);
+      for (int r0 = 0; r0 < nrhs;) r0 += m2lChunk<R, L>(X, s, 0, X.nGroups, r0, nrhs - r0, oc);
     } else if (X.m2lMode == DEFMM_M2L_ON_THE_FLY) {
-      hipLaunchKernelGGL((k_m2l_fly<R, L>), itemGrid(X.nGroups), dim3(TB), 0, s, X.gBeg,
-                         X.gTarget, X.gSrc, X.gMask, X.gCode, (const R2*)ML, (const R2*)X.U, Lb,
-                         oc.twc, oc.tws, X.nGroups);
+      for (int r = 0; r < nrhs; r++)
+        hipLaunchKernelGGL((k_m2l_fly<R, L>), itemGrid(X.nGroups), dim3(M2LShape<L>::TB), 0, s,
+                           X.gBeg, X.gTarget, X.gSrc, X.gMask, X.gCode, (const R2*)Mr(r),
+                           (const R2*)X.U, Lr(r), oc.twc, oc.tws, X.nGroups);
     } else {
       // per batch: its F rows into the workspace, then its groups' M2L
       // (stream order separates a batch's M2L from the next batch's M2F)
       const size_t nb = X.hBatchGroupBeg.size() - 1;
       for (size_t b = 0; b < nb; b++) {
         const int g0 = X.hBatchGroupBeg[b], g1 = X.hBatchGroupBeg[b + 1];
-        const int r0 = X.hBatchRowBeg[b], r1 = X.hBatchRowBeg[b + 1];
-        hipLaunchKernelGGL((k_m2f<R, L>), itemGrid(r1 - r0), dim3(256), 0, s, X.bRows + r0,
-                           (const R2*)ML, F, oc.twc, oc.tws, r1 - r0);
-        hipLaunchKernelGGL((k_m2l<R, L>), itemGrid(g1 - g0), dim3(TB), 0, s, X.gBeg + g0,
-                           X.gTarget + (size_t)g0 * G, X.gSrc, X.gMask, X.gCode, (const R2*)F,
-                           (const R2*)X.U, Lb, oc.twc, oc.tws, g1 - g0);
+        const int w0 = X.hBatchRowBeg[b], w1 = X.hBatchRowBeg[b + 1];
+        for (int r = 0; r < nrhs; r++)
+          hipLaunchKernelGGL((k_m2f<R, L>), itemGrid(w1 - w0), dim3(256), 0, s, X.bRows + w0,
+                             (const R2*)Mr(r), Fr(r), oc.twc, oc.tws, w1 - w0);
+        for (int r0 = 0; r0 < nrhs;) r0 += m2lChunk<R, L>(X, s, g0, g1 - g0, r0, nrhs - r0, oc);
       }
     }
   }
   mark(5);
   if (far && X.nM2LZero)
-    hipLaunchKernelGGL((k_zero_slots<R, L>), dim3(nblk((int64_t)X.nM2LZero * X.LL, 256)),
-                       dim3(256), 0, s, X.m2lZero, X.nM2LZero, Lb);
+    for (int r = 0; r < nrhs; r++)
+      hipLaunchKernelGGL((k_zero_slots<R, L>), dim3(nblk((int64_t)X.nM2LZero * X.LL, 256)),
+                         dim3(256), 0, s, X.m2lZero, X.nM2LZero, Lr(r));
   if (far)
-    for (auto& t : X.l2l)
-      hipLaunchKernelGGL((k_trans<R, L>), itemGrid(t.n), dim3(128), 0, s, t.out, t.beg, t.in,
-                         t.sgn, t.sigIdx, t.edv, (const R2*)Lb, Lb, sig2, sig, oc.B[0], oc.B[1], 1,
-                         t.n);
+    for (int r = 0; r < nrhs; r++)
+      for (auto& t : X.l2l)
+        hipLaunchKernelGGL((k_trans<R, L>), itemGrid(t.n), dim3(128), 0, s, t.out, t.beg, t.in,
+                           t.sgn, t.sigIdx, t.edv, (const R2*)Lr(r), Lr(r), sig2, sig, oc.B[0],
+                           oc.B[1], 1, t.n);
   mark(6);
   if (far && X.nl2p)
-    hipLaunchKernelGGL((k_l2p<R, L>), itemGrid(X.nl2p), dim3(64), 0, s, X.l2p, X.T, X.lv, X.tx,
-                       X.ty, X.tz, X.dirs, sig2, (const R2*)Lb, oc, X.kappa, p, X.nl2p);
+    for (int r = 0; r < nrhs; r++)
+      hipLaunchKernelGGL((k_l2p<R, L>), itemGrid(X.nl2p), dim3(64), 0, s, X.l2p, X.T, X.lv,
+                         X.tx, X.ty, X.tz, X.dirs, sig2, (const R2*)Lr(r), oc, X.kappa, pr(r),
+                         X.nl2p);
   mark(7);
   if (near && X.nitems) {
-    // near field: per-item partial sums, then folded into p per target leaf.
-    // (Run on a second stream beside M2L it measured no faster: both then
-    // contend for the same CUs.)
-    hipLaunchKernelGGL(k_p2p<R>, itemGrid(X.nitems), dim3(64), 0, s, X.items, X.leafCell,
-                       X.T.pbeg, X.T.nprt, X.tx, X.ty, X.tz, X.sx, X.sy, X.sz, (const R2*)q, X.rb,
-                       X.re, X.kappa, X.partial, X.nitems);
-    hipLaunchKernelGGL(k_p2p_reduce<R>, itemGrid(X.nleaves), dim3(64), 0, s, X.itemBeg, X.items,
-                       X.leafCell, X.T.pbeg, X.T.nprt, (const double2*)X.partial, p, X.nleaves);
+    // near field: per-item partial sums, then folded into p per target leaf,
+    // up to 8 right-hand sides per kernel value.  (Run on a second stream
+    // beside M2L it measured no faster: both contend for the same CUs.)
+    for (int r0 = 0; r0 < nrhs;) {
+      const int n = nrhs - r0;
+      if (n >= 8) { launchP2P<R, 8>(X, s, r0); r0 += 8; }
+      else if (n >= 4) { launchP2P<R, 4>(X, s, r0); r0 += 4; }
+      else if (n >= 2) { launchP2P<R, 2>(X, s, r0); r0 += 2; }
+      else { launchP2P<R, 1>(X, s, r0); r0 += 1; }
+    }
   }
   mark(8);
-  hipLaunchKernelGGL(k_scatter<R>, dim3(nblk(X.nt, 256)), dim3(256), 0, s, (const R2*)p, X.tperm,
-                     pout, (int)X.nt);
+  for (int r = 0; r < nrhs; r++)
+    hipLaunchKernelGGL(k_scatter<R>, dim3(nblk(X.nt, 256)), dim3(256), 0, s, (const R2*)pr(r),
+                       X.tperm, pout + r * nt, (int)X.nt);
   mark(9);
   X.evRecorded = true;
   HIPCK(hipGetLastError());
@@ -1608,24 +1735,32 @@ static int runTyped(Exec& X, const double2* qin, double2* pout, int parts, hipSt
 }
 
 template <typename R>
-static int runR(Exec& X, const double2* qin, double2* pout, int parts, hipStream_t s) {
+static int runR(Exec& X, const double2* qin, double2* pout, int parts, int nrhs, hipStream_t s) {
   switch (X.L) {
-    case 2: return runTyped<R, 2>(X, qin, pout, parts, s);
-    case 3: return runTyped<R, 3>(X, qin, pout, parts, s);
-    case 4: return runTyped<R, 4>(X, qin, pout, parts, s);
-    case 5: return runTyped<R, 5>(X, qin, pout, parts, s);
-    case 6: return runTyped<R, 6>(X, qin, pout, parts, s);
-    case 7: return runTyped<R, 7>(X, qin, pout, parts, s);
-    case 8: return runTyped<R, 8>(X, qin, pout, parts, s);
+    case 2: return runTyped<R, 2>(X, qin, pout, parts, nrhs, s);
+    case 3: return runTyped<R, 3>(X, qin, pout, parts, nrhs, s);
+    case 4: return runTyped<R, 4>(X, qin, pout, parts, nrhs, s);
+    case 5: return runTyped<R, 5>(X, qin, pout, parts, nrhs, s);
+    case 6: return runTyped<R, 6>(X, qin, pout, parts, nrhs, s);
+    case 7: return runTyped<R, 7>(X, qin, pout, parts, nrhs, s);
+    case 8: return runTyped<R, 8>(X, qin, pout, parts, nrhs, s);
   }
   g_err = "interpolation order outside 2..8";
   return ERR_ARG;
 }
 
-static int run(Exec& X, const double2* qin, double2* pout, int parts, hipStream_t s) {
+// nrhs right-hand sides in chunks of the plan's width
+static int run(Exec& X, const double2* qin, double2* pout, int parts, int nrhs, hipStream_t s) {
   HIPCK(hipSetDevice(X.device));
-  return X.prec == DEFMM_FP64 ? runR<double>(X, qin, pout, parts, s)
-                              : runR<float>(X, qin, pout, parts, s);
+  for (int r0 = 0; r0 < nrhs; r0 += X.nr) {
+    const int n = std::min(X.nr, nrhs - r0);
+    const double2* q = qin + (size_t)r0 * X.ns;
+    double2* p = pout + (size_t)r0 * X.nt;
+    int rc = X.prec == DEFMM_FP64 ? runR<double>(X, q, p, parts, n, s)
+                                  : runR<float>(X, q, p, parts, n, s);
+    if (rc) return rc;
+  }
+  return OK;
 }
 
 }  // namespace dfm
@@ -1698,32 +1833,67 @@ int defmm_plan_create(const double* src_xyz, int64_t n_src, const double* trg_xy
 
 int defmm_execute_device(defmm_plan plan, const void* q_dev, void* p_dev, void* stream) {
   if (!plan || !q_dev || !p_dev) { g_err = "null argument"; return DEFMM_EARG; }
-  return dfm::run(plan->x, (const double2*)q_dev, (double2*)p_dev, 3, (hipStream_t)stream);
+  return dfm::run(plan->x, (const double2*)q_dev, (double2*)p_dev, 3, 1, (hipStream_t)stream);
 }
 
-static int execHost(defmm_plan plan, const double* q, double* p, int parts) {
+int defmm_execute_batch_device(defmm_plan plan, const void* q_dev, void* p_dev, int nrhs,
+                               void* stream) {
+  if (!plan || !q_dev || !p_dev) { g_err = "null argument"; return DEFMM_EARG; }
+  if (nrhs < 1) { g_err = "nrhs must be positive"; return DEFMM_EARG; }
+  return dfm::run(plan->x, (const double2*)q_dev, (double2*)p_dev, 3, nrhs, (hipStream_t)stream);
+}
+
+static int execHost(defmm_plan plan, const double* q, double* p, int parts, int nrhs) {
   if (!plan || !q || !p) { g_err = "null argument"; return DEFMM_EARG; }
   if (parts < 1 || parts > 3) { g_err = "parts must be 1, 2 or 3"; return DEFMM_EARG; }
+  if (nrhs < 1) { g_err = "nrhs must be positive"; return DEFMM_EARG; }
   Exec& X = plan->x;
   if (hipSetDevice(X.device) != hipSuccess) { g_err = "hipSetDevice failed"; return DEFMM_EHIP; }
-  if (hipMemcpyAsync(X.qio, q, X.ns * sizeof(double2), hipMemcpyHostToDevice, X.own) != hipSuccess) {
-    g_err = "copy of charges failed";
-    return DEFMM_EHIP;
-  }
-  int rc = dfm::run(X, X.qio, X.pio, parts, X.own);
-  if (rc) return rc;
-  if (hipMemcpyAsync(p, X.pio, X.nt * sizeof(double2), hipMemcpyDeviceToHost, X.own) != hipSuccess ||
-      hipStreamSynchronize(X.own) != hipSuccess) {
-    g_err = std::string("execution failed: ") + hipGetErrorString(hipGetLastError());
-    return DEFMM_EHIP;
+  for (int r0 = 0; r0 < nrhs; r0 += X.nr) {
+    const int n = std::min(X.nr, nrhs - r0);
+    if (hipMemcpyAsync(X.qio, q + 2 * (size_t)r0 * X.ns, n * X.ns * sizeof(double2),
+                       hipMemcpyHostToDevice, X.own) != hipSuccess) {
+      g_err = "copy of charges failed";
+      return DEFMM_EHIP;
+    }
+    int rc = dfm::run(X, X.qio, X.pio, parts, n, X.own);
+    if (rc) return rc;
+    if (hipMemcpyAsync(p + 2 * (size_t)r0 * X.nt, X.pio, n * X.nt * sizeof(double2),
+                       hipMemcpyDeviceToHost, X.own) != hipSuccess ||
+        hipStreamSynchronize(X.own) != hipSuccess) {
+      g_err = std::string("execution failed: ") + hipGetErrorString(hipGetLastError());
+      return DEFMM_EHIP;
+    }
   }
   return DEFMM_OK;
 }
 
-int defmm_execute(defmm_plan plan, const double* q, double* p) { return execHost(plan, q, p, 3); }
+int defmm_execute(defmm_plan plan, const double* q, double* p) {
+  return execHost(plan, q, p, 3, 1);
+}
+
+int defmm_execute_batch(defmm_plan plan, const double* q, double* p, int nrhs) {
+  return execHost(plan, q, p, 3, nrhs);
+}
 
 int defmm_execute_parts(defmm_plan plan, const double* q, double* p, int parts) {
-  return execHost(plan, q, p, parts);
+  return execHost(plan, q, p, parts, 1);
+}
+
+int defmm_plan_set_rhs(defmm_plan plan, int nrhs) {
+  if (!plan) { g_err = "null plan"; return DEFMM_EARG; }
+  if (nrhs < 1 || nrhs > 64) { g_err = "nrhs must be in 1..64"; return DEFMM_EARG; }
+  Exec& X = plan->x;
+  if (nrhs == X.nr) return DEFMM_OK;
+  if (hipSetDevice(X.device) != hipSuccess || hipStreamSynchronize(X.own) != hipSuccess) {
+    g_err = "device synchronisation failed";
+    return DEFMM_EHIP;
+  }
+  X.nr = nrhs;
+  X.evRecorded = false;
+  X.freeTyped();
+  int rc = dfm::chooseM2LMode(X);
+  return rc ? rc : dfm::uploadPrecision(X);
 }
 
 int defmm_plan_stats(defmm_plan plan, int64_t* out, int n) {
@@ -1781,6 +1951,7 @@ int defmm_plan_set_m2l(defmm_plan plan, int mode) {
     return DEFMM_EHIP;
   }
   X.m2lMode = mode;
+  X.m2lAuto = false;
   X.evRecorded = false;
   return dfm::uploadPrecision(X);
 }

@@ -72,6 +72,27 @@ int defmm_execute(defmm_plan plan, const double* q, double* p);
 int defmm_execute_device(defmm_plan plan, const void* q_dev, void* p_dev, void* stream);
 
 /*
+ * Several right-hand sides at once (the reference evaluates one per gemv):
+ * q holds nrhs x n_src complex charges, right-hand side after right-hand
+ * side (q[2*(r*n_src + i)] = Re q_r,i), p likewise nrhs x n_trg.  The pass
+ * handles up to defmm_plan_set_rhs() right-hand sides at a time, M2L and P2P
+ * up to 8 per kernel (one table or kernel value for all of them).  Host
+ * arrays, synchronous.
+ */
+int defmm_execute_batch(defmm_plan plan, const double* q, double* p, int nrhs);
+
+/* The same with device arrays, enqueued on stream (NULL = default stream). */
+int defmm_execute_batch_device(defmm_plan plan, const void* q_dev, void* p_dev, int nrhs,
+                               void* stream);
+
+/*
+ * Width of the expansion stores in right-hand sides (1..64, default 1):
+ * reallocates them; if the M2L variant was not set explicitly, re-chooses
+ * it for the new width (stored F when it fits, else batched).
+ */
+int defmm_plan_set_rhs(defmm_plan plan, int nrhs);
+
+/*
  * Evaluation of a part of the interaction, for verification: parts bit 0 =
  * near field (P2P), bit 1 = far field (P2M ... L2P).  parts = 3 is
  * defmm_execute.  Host memory, synchronous.

@@ -164,6 +164,32 @@ def test_m2l_variants_match_oracle(cube, cube_charges, mode, rows):
         assert l2 <= FP64_L2 and mx <= FP64_MAX
 
 
+@pytest.mark.parametrize("mode", [1, 2, 0])
+def test_multi_rhs_matches_single(cube, mode):
+    """Several right-hand sides per pass (M2L and P2P up to 8 per kernel)
+    give each right-hand side's single evaluation: 7 right-hand sides with a
+    width of 5 (chunks 5 + 2: kernels of 4, 2 and 1) and of 8 (4 + 2 + 1)."""
+    dm = _gpu()
+    from defmm_amd import workloads as wl
+    nr = 7
+    Q = np.stack([wl.charges(len(cube), seed=40 + r) for r in range(nr)])
+    for prec, tol in ((dm.FP64, 1e-13), (dm.MIXED, 1e-6)):
+        _say(f"case: refined cube L=4, {nr} right-hand sides, M2L mode {mode}, precision {prec}")
+        A = dm.IBFMM_Mat(precision=prec)
+        A.setSourceParticles(cube)
+        A.setTargetParticles(cube)
+        A.prcmpt(4, 32, 1.0)
+        A.set_m2l(mode)
+        single = np.stack([A.apply(Q[r]) for r in range(nr)])
+        for width in (5, 8):
+            A.set_rhs(width)
+            P = A.apply_batch(Q)
+            l2, mx = _diff(P, single)
+            _say(f"  width {width}: batch vs single rel L2 {l2:.3e} max {mx:.3e}")
+            assert l2 <= tol and mx <= 10 * tol
+        A.destroy()
+
+
 def test_repeated_execution_is_identical(cube, cube_charges):
     """The reference accumulates into buffers zeroed only at prcmpt, so its
     second gemv is wrong; a plan here can be executed again."""
