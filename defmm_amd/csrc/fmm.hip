@@ -363,17 +363,15 @@ __global__ __launch_bounds__(256) void k_m2f(const int* __restrict__ m2fSlot,
 //   L_g[I] = (1/PP) * IDFT(J_g)[I2i(I, P)]          (written, not accumulated)
 // Each entry's F row is loaded once for all targets of the group that read it.
 // J never leaves LDS; the inverse transform only produces the L^3 corner.
-// perm_h (permutations.hpp:127-150) is evaluated arithmetically: with
-// h = z + 8 s, component k of the permuted index is w[sigma_s[k]], replaced
-// by P - w[sigma_s[k]] when bit sigma_s[k] of z is set and it is nonzero.
-// The kernel tables are stored in the 6 axis orders of the symmetry group:
-// variant v of table t holds V[u] = U_t[(u_{s0}, u_{s1}, u_{s2})] for the axis
-// permutation s = sigma(v) (permutations.hpp:87-120).  Then
-// U_t[perm_h(w)] = V[flip_z(w)] with h = z + 8 s (permutations.hpp:127-150):
-// component j of w is replaced by P - w_j when bit j of z is set and w_j != 0.
-// Only sign flips remain, which keep a row of w contiguous (reversed), so the
-// gather reads whole cache lines.  The device code of a (table, h) pair is
-// ((t * 6 + v(s)) << 3) | z: the row of the table variant and the flips.
+// perm_h (permutations.hpp:127-150): with h = z + 8 s, component k of the
+// permuted index is w[sigma_s[k]], replaced by P - w[sigma_s[k]] when bit
+// sigma_s[k] of z is set and it is nonzero.  Rather than permuting per
+// frequency, the table store holds one row per (table t, axis order s, flips
+// z) that the plan uses, row[w] = U_t[perm_h(w)], so every load reads
+// frequency w of a row at a fixed per-thread offset (at 1M points / 128
+// wavelengths / L = 5: 779,700 rows, 9.1 GB in FP64).  The device code of an
+// entry's target is its row << 3.  kVariantOfSigma maps the 3 bits of sigma
+// in h to the axis-order index s (permutations.hpp:87-120).
 static const int kVariantOfSigma[8] = {0, 1, 0, 2, 3, 0, 4, 5};
 
 // M2L block shape for order L: NW frequencies per thread (at most 3 where a
@@ -407,17 +405,11 @@ constexpr int m2lMaxNR() {
   return cap >= 8 ? 8 : cap >= 4 ? 4 : cap >= 2 ? 2 : 1;
 }
 
-template <int P>
-__device__ __forceinline__ int flipIndex(int z, int w0, int w1, int w2) {
-  const int a0 = ((z & 1) && w0) ? P - w0 : w0;
-  const int a1 = ((z & 2) && w1) ? P - w1 : w1;
-  const int a2 = ((z & 4) && w2) ? P - w2 : w2;
-  return (a0 * P + a1) * P + a2;
-}
 template <int PP>
 __device__ __forceinline__ size_t tableRow(int code) {
   return (size_t)(code >> 3) * PP;
 }
+
 
 // NR right-hand sides per launch: right-hand side r reads F + r * fStride
 // and writes Lout + r * lStride; each table value is loaded once for all.
@@ -448,15 +440,9 @@ __global__ __launch_bounds__((M2LShapeN<L, NR>::TB)) void k_m2l(
   }
   // frequencies of this thread, clamped to PP - 1: every load is in bounds
   // and unconditional, and the accumulators of w >= PP are never stored
-  int wl[NW], w0[NW], w1[NW], w2[NW];
+  unsigned wl[NW];
 #pragma unroll
-  for (int m = 0; m < NW; m++) {
-    const int w = min(tid + TB * m, PP - 1);
-    wl[m] = w;
-    w0[m] = w / (P * P);
-    w1[m] = (w / P) % P;
-    w2[m] = w % P;
-  }
+  for (int m = 0; m < NW; m++) wl[m] = (unsigned)min(tid + TB * m, PP - 1);
   R2 acc[G][NW][NR];
 #pragma unroll
   for (int g = 0; g < G; g++)
@@ -479,14 +465,11 @@ __global__ __launch_bounds__((M2LShapeN<L, NR>::TB)) void k_m2l(
       for (int r = 0; r < NR; r++) f[m][r] = Fr[r * fStride + wl[m]];
 #pragma unroll
     for (int g = 0; g < G; g++) {
-      const R2* Ur = U + tableRow<PP>(code[g]);
-      const int z = code[g] & 7;
+      // (u of a target absent from the entry is left unset and not read)
       if ((mask >> g) & 1) {
+        const R2* Ur = U + tableRow<PP>(code[g]);
 #pragma unroll
-        for (int m = 0; m < NW; m++) u[g][m] = Ur[flipIndex<P>(z, w0[m], w1[m], w2[m])];
-      } else {
-#pragma unroll
-        for (int m = 0; m < NW; m++) u[g][m] = c2<R>(0, 0);
+        for (int m = 0; m < NW; m++) u[g][m] = Ur[wl[m]];
       }
     }
 #pragma unroll
@@ -591,13 +574,12 @@ __global__ __launch_bounds__(M2LShape<L>::TB) void k_m2l_fly(const int* __restri
     wc[tid] = twc[tid];
     ws[tid] = tws[tid];
   }
-  int w0[NW], w1[NW], w2[NW];
+  unsigned wl[NW];
+  int w0[NW];
 #pragma unroll
   for (int m = 0; m < NW; m++) {
-    const int w = min(tid + TB * m, PP - 1);
-    w0[m] = w / (P * P);
-    w1[m] = (w / P) % P;
-    w2[m] = w % P;
+    wl[m] = (unsigned)min(tid + TB * m, PP - 1);
+    w0[m] = (int)wl[m] / (P * P);
   }
   R2 acc[G][NW];
 #pragma unroll
@@ -613,12 +595,10 @@ __global__ __launch_bounds__(M2LShape<L>::TB) void k_m2l_fly(const int* __restri
     R2 u[G][NW];
 #pragma unroll
     for (int g = 0; g < G; g++) {
-      const int code = gCode[(size_t)e * G + g];
-      const R2* Ur = U + tableRow<PP>(code);
-      const int z = code & 7;
+      const R2* Ur = U + tableRow<PP>(gCode[(size_t)e * G + g]);
       if ((mask >> g) & 1) {
 #pragma unroll
-        for (int m = 0; m < NW; m++) u[g][m] = Ur[flipIndex<P>(z, w0[m], w1[m], w2[m])];
+        for (int m = 0; m < NW; m++) u[g][m] = Ur[wl[m]];
       } else {
 #pragma unroll
         for (int m = 0; m < NW; m++) u[g][m] = c2<R>(0, 0);
@@ -1161,6 +1141,7 @@ struct Exec {
   double stageMs[DEFMM_NSTAGES] = {};
   // host copies needed to re-upload the typed buffers in another precision
   std::vector<cplx> hSig, hSig2, hTab;
+  std::vector<int> hRowKey;  // per table-store row: (table * 6 + axis order) * 8 + flips
   std::vector<double> hA[2], hB[2];
   size_t nMLslots = 0;
   std::vector<void*> typedMem;
@@ -1308,21 +1289,31 @@ static int uploadTyped(Exec& X) {
     a = toR2<R>(X.hSig2);
     CKR(X.upv(d, a));
     X.sig2 = d;
-    // 6 axis orders per table (see tableRow)
+    // one row per (table, axis order s, flips z) in use:
+    // row[w] = U_t[(a_{s0}, a_{s1}, a_{s2})] with a = flip_z(w)
     static const int kVarSigma[6][3] = {{0, 1, 2}, {1, 0, 2}, {1, 2, 0},
                                         {0, 2, 1}, {2, 0, 1}, {2, 1, 0}};
     const int P = X.P, PP = X.PP;
     const size_t ntab = X.hTab.size() / (size_t)PP;
-    std::vector<R2> u6(ntab * 6 * (size_t)PP);
-    for (size_t t = 0; t < ntab; t++)
-      for (int v = 0; v < 6; v++)
-        for (int i = 0; i < PP; i++) {
-          int u[3] = {i / (P * P), (i / P) % P, i % P};
-          int src = (u[kVarSigma[v][0]] * P + u[kVarSigma[v][1]]) * P + u[kVarSigma[v][2]];
-          const cplx& c = X.hTab[t * PP + src];
-          u6[(t * 6 + v) * PP + i] = c2h<R>(c.real(), c.imag());
-        }
-    CKR(X.upv(d, u6));
+    std::vector<R2> rowsU(X.hRowKey.size() * (size_t)PP, c2h<R>(0., 0.));
+    // (a plan without M2L keeps one zero row that is never read)
+    for (size_t k = 0; k < X.hRowKey.size() && ntab > 0; k++) {
+      const int key = X.hRowKey[k];
+      const int t = key / 48, v = (key / 8) % 6, z = key % 8;
+      if ((size_t)t >= ntab) {
+        g_err = "internal: M2L table id out of range";
+        return ERR_STATE;
+      }
+      for (int i = 0; i < PP; i++) {
+        int a[3] = {i / (P * P), (i / P) % P, i % P};
+        for (int j = 0; j < 3; j++)
+          if (((z >> j) & 1) && a[j]) a[j] = P - a[j];
+        const int src = (a[kVarSigma[v][0]] * P + a[kVarSigma[v][1]]) * P + a[kVarSigma[v][2]];
+        const cplx& c = X.hTab[t * PP + src];
+        rowsU[k * PP + i] = c2h<R>(c.real(), c.imag());
+      }
+    }
+    CKR(X.upv(d, rowsU));
     X.U = d;
   }
   {
@@ -1488,12 +1479,30 @@ static int upload(Exec& X, Plan& pl) {
   CKR(X.upv(X.gTarget, pl.m2lGroupTarget));
   CKR(X.upv(X.gMask, pl.m2lGMask));
   {
-    // (table id << 6 | h) -> (table variant row << 3 | flips)
-    std::vector<int> dc(pl.m2lGCode.size());
-    for (size_t k = 0; k < dc.size(); k++) {
-      const int c = pl.m2lGCode[k];
-      dc[k] = (((c >> 6) * 6 + kVariantOfSigma[(c >> 3) & 7]) << 3) | (c & 7);
-    }
+    // (table id << 6 | h) -> row of the materialised table store << 3: one
+    // row per (table, axis order, flips) that some entry reads, in order of
+    // first use (codes of absent group targets point at row 0, never read)
+    const size_t nkeys = (size_t)std::max(1, pl.ntab) * 48;
+    std::vector<int> rowOf(nkeys, -1);
+    X.hRowKey.clear();
+    std::vector<int> dc(pl.m2lGCode.size(), 0);
+    const int G = pl.m2lG;
+    for (size_t e = 0; e < pl.m2lGMask.size(); e++)
+      for (int g = 0; g < G; g++) {
+        if (!((pl.m2lGMask[e] >> g) & 1)) continue;
+        const int c = pl.m2lGCode[e * G + g];
+        const size_t key = ((size_t)(c >> 6) * 6 + kVariantOfSigma[(c >> 3) & 7]) * 8 + (c & 7);
+        if (key >= nkeys) {
+          g_err = "internal: M2L table code out of range";
+          return ERR_STATE;
+        }
+        if (rowOf[key] < 0) {
+          rowOf[key] = (int)X.hRowKey.size();
+          X.hRowKey.push_back((int)key);
+        }
+        dc[e * G + g] = rowOf[key] << 3;
+      }
+    if (X.hRowKey.empty()) X.hRowKey.push_back(0);
     CKR(X.upv(X.gCode, dc));
   }
   X.hGSrcF.swap(pl.m2lGSrcF);
