@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/defmm_amd.h"
@@ -825,27 +826,47 @@ struct P2PItem {
 
 // Kernel value G = exp(i kappa r) / (4 pi r) between a target and a source
 // (P2P.hpp:61-75), 0 when r^2 < 1e-16.
-template <typename R>
+// Small: kappa * r <= 2^20 for every pair (checked on the host), so the
+// double sine and cosine need no range check.
+template <typename R, bool Small>
 __device__ __forceinline__ void p2pKernel(double xt, double yt, double zt, double xs, double ys,
                                           double zs, double kappa, R& co, R& si);
 template <>
-__device__ __forceinline__ void p2pKernel<double>(double xt, double yt, double zt, double xs,
-                                                  double ys, double zs, double kappa,
-                                                  double& co, double& si) {
+__device__ __forceinline__ void p2pKernel<double, true>(double xt, double yt, double zt,
+                                                        double xs, double ys, double zs,
+                                                        double kappa, double& co, double& si) {
   const double x = xt - xs, y = yt - ys, z = zt - zs;
   const double R2v = x * x + y * y + z * z;
   double K = rsqrt(R2v);
   if (R2v < 1.e-16) K = 0.;
   const double r = K * R2v * kappa;
   K *= 0.079577471545947667884;  // 1/(4 pi)
-  sincos(r, &si, &co);
+  sincosRedSmall(r, si, co);
   co *= K;
   si *= K;
 }
 template <>
-__device__ __forceinline__ void p2pKernel<float>(double xt, double yt, double zt, double xs,
-                                                 double ys, double zs, double kappa, float& co,
-                                                 float& si) {
+__device__ __forceinline__ void p2pKernel<double, false>(double xt, double yt, double zt,
+                                                         double xs, double ys, double zs,
+                                                         double kappa, double& co, double& si) {
+  const double x = xt - xs, y = yt - ys, z = zt - zs;
+  const double R2v = x * x + y * y + z * z;
+  double K = rsqrt(R2v);
+  if (R2v < 1.e-16) K = 0.;
+  const double r = K * R2v * kappa;
+  K *= 0.079577471545947667884;  // 1/(4 pi)
+  sincosRed(r, si, co);
+  co *= K;
+  si *= K;
+}
+template <>
+__device__ __forceinline__ void p2pKernel<float, true>(double xt, double yt, double zt,
+                                                       double xs, double ys, double zs,
+                                                       double kappa, float& co, float& si);
+template <>
+__device__ __forceinline__ void p2pKernel<float, false>(double xt, double yt, double zt,
+                                                        double xs, double ys, double zs,
+                                                        double kappa, float& co, float& si) {
   const double x = xt - xs, y = yt - ys, z = zt - zs;
   const double R2v = x * x + y * y + z * z;
   // r to ~1e-14 relative: float reciprocal square root, one Newton step in double
@@ -856,6 +877,12 @@ __device__ __forceinline__ void p2pKernel<float>(double xt, double yt, double zt
   expi(kappa * r, co, si);
   co *= K;
   si *= K;
+}
+template <>
+__device__ __forceinline__ void p2pKernel<float, true>(double xt, double yt, double zt,
+                                                       double xs, double ys, double zs,
+                                                       double kappa, float& co, float& si) {
+  p2pKernel<float, false>(xt, yt, zt, xs, ys, zs, kappa, co, si);
 }
 
 // NR right-hand sides: charges q + r * qStride; the partial sums of item k,
@@ -874,8 +901,8 @@ __global__ __launch_bounds__(64) void k_p2p(const P2PItem* __restrict__ items,
                                             const typename C2T<R>::T* __restrict__ q,
                                             size_t qStride,
                                             const int* __restrict__ rb, const int* __restrict__ re,
-                                            double kappa, double2* __restrict__ partial,
-                                            int nItems) {
+                                            double kappa, int smallArg,
+                                            double2* __restrict__ partial, int nItems) {
   typedef typename C2T<R>::T R2;
   __shared__ double lx[64], ly[64], lz[64];
   __shared__ R2 lq[NR][64];
@@ -932,15 +959,21 @@ __global__ __launch_bounds__(64) void k_p2p(const P2PItem* __restrict__ items,
       R pr[NR], pi[NR];
 #pragma unroll
       for (int k = 0; k < NR; k++) pr[k] = pi[k] = 0;
-      for (int j = g; j < tileN; j += G) {
+      auto term = [&](int j, auto small) {
         R co, si;
-        p2pKernel<R>(xt, yt, zt, lx[j], ly[j], lz[j], kappa, co, si);
+        p2pKernel<R, decltype(small)::value>(xt, yt, zt, lx[j], ly[j], lz[j], kappa, co, si);
 #pragma unroll
         for (int k = 0; k < NR; k++) {
           const R2 qs = lq[k][j];
           pr[k] += co * qs.x - si * qs.y;
           pi[k] += co * qs.y + si * qs.x;
         }
+      };
+      if (smallArg) {
+#pragma unroll 2
+        for (int j = g; j < tileN; j += G) term(j, std::true_type());
+      } else {
+        for (int j = g; j < tileN; j += G) term(j, std::false_type());
       }
 #pragma unroll
       for (int k = 0; k < NR; k++) {
@@ -1128,6 +1161,7 @@ struct Exec {
   void* Lb = nullptr;        // local expansions (aliases ML when F is stored)
   size_t mStride = 0, lStride = 0, fStride = 0;  // elements between right-hand sides
   int m2lMode = DEFMM_M2L_STORED;
+  int smallArg = 0;          // kappa * (largest point distance) <= 2^20
   bool m2lAuto = true;       // mode chosen by free memory (until set explicitly)
   int nr = 1;                // right-hand sides the expansion stores hold
   int batchRows = 16384;     // requested workspace rows of the batched M2L
@@ -1413,6 +1447,22 @@ static int uploadPrecision(Exec& X) {
 
 static int upload(Exec& X, Plan& pl) {
   X.L = pl.L; X.P = pl.P; X.LL = pl.LL; X.PP = pl.PP; X.kappa = pl.kappa;
+  {
+    // every near-field phase kappa * r is below kappa times the diagonal of
+    // the box holding all points
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (const Tree* t : {&pl.S, &pl.T})
+      for (size_t i = 0; i < t->x.size(); i++) {
+        const double v[3] = {t->x[i], t->y[i], t->z[i]};
+        for (int d = 0; d < 3; d++) {
+          lo[d] = std::min(lo[d], v[d]);
+          hi[d] = std::max(hi[d], v[d]);
+        }
+      }
+    double diag2 = 0.;
+    for (int d = 0; d < 3; d++) diag2 += (hi[d] - lo[d]) * (hi[d] - lo[d]);
+    X.smallArg = std::fabs(pl.kappa) * std::sqrt(diag2) * (1. + 1e-12) <= 1048576.0 ? 1 : 0;
+  }
   X.st = pl.st;
   X.ns = (int64_t)pl.S.x.size();
   X.nt = (int64_t)pl.T.x.size();
@@ -1610,7 +1660,7 @@ static void launchP2P(Exec& X, hipStream_t s, int r0) {
   R2* p = (R2*)X.p + (size_t)r0 * X.nt;
   hipLaunchKernelGGL((k_p2p<R, NR>), itemGrid(X.nitems), dim3(64), 0, s, X.items, X.leafCell,
                      X.T.pbeg, X.T.nprt, X.tx, X.ty, X.tz, X.sx, X.sy, X.sz, q, (size_t)X.ns, X.rb,
-                     X.re, X.kappa, X.partial, X.nitems);
+                     X.re, X.kappa, X.smallArg, X.partial, X.nitems);
   hipLaunchKernelGGL((k_p2p_reduce<R, NR>), itemGrid(X.nleaves), dim3(64), 0, s, X.itemBeg,
                      X.items, X.leafCell, X.T.pbeg, X.T.nprt, (const double2*)X.partial, p,
                      (size_t)X.nt, X.nleaves);

@@ -7,6 +7,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "sincos_red.hpp"
+
 namespace dfm {
 
 constexpr int LMAX = 8;  // largest interpolation order supported on device
@@ -38,10 +40,10 @@ __device__ __forceinline__ A cmulc(A a, B b) {
   return r;
 }
 
-// exp(i*arg).  double: direct.  float: the argument is reduced modulo 2*pi
-// in double first, so that a float evaluation keeps an absolute phase error
-// near 1e-7 for arguments of hundreds of radians.
-__device__ __forceinline__ void expi(double arg, double& c, double& s) { sincos(arg, &s, &c); }
+// exp(i*arg).  double: sincosRed (sincos_red.hpp).  float: the argument is
+// reduced modulo 2*pi in double first, so that a float evaluation keeps an
+// absolute phase error near 1e-7 for arguments of hundreds of radians.
+__device__ __forceinline__ void expi(double arg, double& c, double& s) { sincosRed(arg, s, c); }
 __device__ __forceinline__ void expi(double arg, float& c, float& s) {
   const double inv2pi = 0.15915494309189533577;
   double t = arg * inv2pi;
