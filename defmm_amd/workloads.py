@@ -85,6 +85,19 @@ def ellipsoid_clusters(n, seed=1, axes=(1.0, 0.25, 0.25), ncentres=8, sigma=0.05
     return x / np.linalg.norm(x / a, axis=1, keepdims=True)
 
 
+def gaussian_clusters(n, seed=1, ncentres=16, sigma=0.05):
+    """Gaussian clusters in the unit cube: centres uniform in [0, 1]^3, points
+    centre + sigma * N(0, I3)."""
+    c = np.stack([uniform(seed + 201 + k, ncentres) for k in range(3)], axis=1)
+    k = np.minimum((uniform(seed + 9, n) * ncentres).astype(np.int64), ncentres - 1)
+    g = np.stack([normal(seed + 17 * j + 5, n) for j in range(3)], axis=1)
+    return c[k] + sigma * g
+
+
+GEOMETRIES = {"sphere": sphere, "cube_edges": cube_edges, "ellipsoid": ellipsoid_clusters,
+              "clusters": gaussian_clusters}
+
+
 def charges(n, seed=7):
     """Complex charges with re, im ~ U(-1, 1)."""
     return (2 * uniform(seed, n) - 1) + 1j * (2 * uniform(seed + 1, n) - 1)
@@ -112,5 +125,5 @@ CONFIGS = {
 
 def make(name, seed=1):
     geo, n, nl, L, ncrit = CONFIGS[name]
-    pts = {"cube_edges": cube_edges, "sphere": sphere, "ellipsoid": ellipsoid_clusters}[geo](n, seed)
+    pts = GEOMETRIES[geo](n, seed)
     return pts, kappa_for(nl, pts), L, ncrit
