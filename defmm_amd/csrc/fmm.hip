@@ -198,8 +198,13 @@ __global__ __launch_bounds__(128) void k_trans(
     mt[0][x] = mat0[x];
     mt[1][x] = mat1[x];
   }
-  const int o = itemIndex();
-  if (o >= nItems) return;
+  // consecutive outputs (the directions of one cell) read the same input
+  // slots: give each XCD a contiguous part of the output order so that they
+  // meet in one L2 (bijection of [0, n8), identity on the tail)
+  const int blk = itemIndex();
+  if (blk >= nItems) return;
+  const int n8 = nItems & ~7;
+  const int o = blk < n8 ? (blk & 7) * (n8 >> 3) + (blk >> 3) : blk;
   const int e0 = beg[o], e1 = beg[o + 1];
   R2 acc[NI];
 #pragma unroll
@@ -296,64 +301,73 @@ __global__ __launch_bounds__(128) void k_trans(
 // the (2L-1)^3 grid at I2i(I, P), unnormalised, exp(-2 pi i jk/P).  The
 // embedding is zero outside the L^3 corner, so the three axis passes sum over
 // L inputs only.
+// kM2FRows rows per block: their M rows are loaded together at the start (the
+// single-row kernel spent most of its time waiting for that load), then the
+// rows are transformed one after another in LDS.
+constexpr int kM2FRows = 4;
 template <typename R, int L>
 __global__ __launch_bounds__(256) void k_m2f(const int* __restrict__ m2fSlot,
                                              const typename C2T<R>::T* __restrict__ M,
                                              typename C2T<R>::T* __restrict__ F,
                                              const R* __restrict__ twc,
-                                             const R* __restrict__ tws, int nItems) {
+                                             const R* __restrict__ tws, int nRows) {
   typedef typename C2T<R>::T R2;
   constexpr int P = 2 * L - 1, LL = L * L * L, PP = P * P * P;
-  __shared__ R2 sM[LL], T1[L * L * P], T2[L * P * P];
+  __shared__ R2 sM[kM2FRows][LL], T1[L * L * P], T2[L * P * P];
   __shared__ R wc[P], ws[P];
   const int tid = threadIdx.x;
-  const int item = itemIndex();
-  if (item >= nItems) return;
+  const int row0 = itemIndex() * kM2FRows;
+  if (row0 >= nRows) return;
+  const int nr = min(kM2FRows, nRows - row0);
   if (tid < P) {
     wc[tid] = twc[tid];
     ws[tid] = tws[tid];
   }
-  const int s = m2fSlot[item];
-  for (int I = tid; I < LL; I += 256) sM[I] = M[(size_t)s * LL + I];
-  __syncthreads();
-  for (int x = tid; x < L * L * P; x += 256) {
-    const int k2 = x % P, r = x / P;
-    R2 t = c2<R>(0, 0);
-#pragma unroll
-    for (int j = 0; j < L; j++) {
-      const int m = (j * k2) % P;
-      const R2 v = sM[r * L + j];
-      t.x += v.x * wc[m] + v.y * ws[m];
-      t.y += v.y * wc[m] - v.x * ws[m];
-    }
-    T1[x] = t;
+  for (int x = tid; x < nr * LL; x += 256) {
+    const int k = x / LL, I = x - k * LL;
+    sM[k][I] = M[(size_t)m2fSlot[row0 + k] * LL + I];
   }
-  __syncthreads();
-  for (int x = tid; x < L * P * P; x += 256) {
-    const int k2 = x % P, k1 = (x / P) % P, i0 = x / (P * P);
-    R2 t = c2<R>(0, 0);
+  for (int k = 0; k < nr; k++) {
+    __syncthreads();
+    for (int x = tid; x < L * L * P; x += 256) {
+      const int k2 = x % P, r = x / P;
+      R2 t = c2<R>(0, 0);
 #pragma unroll
-    for (int j = 0; j < L; j++) {
-      const int m = (j * k1) % P;
-      const R2 v = T1[(i0 * L + j) * P + k2];
-      t.x += v.x * wc[m] + v.y * ws[m];
-      t.y += v.y * wc[m] - v.x * ws[m];
+      for (int j = 0; j < L; j++) {
+        const int m = (j * k2) % P;
+        const R2 v = sM[k][r * L + j];
+        t.x += v.x * wc[m] + v.y * ws[m];
+        t.y += v.y * wc[m] - v.x * ws[m];
+      }
+      T1[x] = t;
     }
-    T2[x] = t;
-  }
-  __syncthreads();
-  R2* Fr = F + (size_t)item * PP;
-  for (int x = tid; x < PP; x += 256) {
-    const int k0 = x / (P * P), r = x % (P * P);
-    R2 t = c2<R>(0, 0);
+    __syncthreads();
+    for (int x = tid; x < L * P * P; x += 256) {
+      const int k2 = x % P, k1 = (x / P) % P, i0 = x / (P * P);
+      R2 t = c2<R>(0, 0);
 #pragma unroll
-    for (int j = 0; j < L; j++) {
-      const int m = (j * k0) % P;
-      const R2 v = T2[j * P * P + r];
-      t.x += v.x * wc[m] + v.y * ws[m];
-      t.y += v.y * wc[m] - v.x * ws[m];
+      for (int j = 0; j < L; j++) {
+        const int m = (j * k1) % P;
+        const R2 v = T1[(i0 * L + j) * P + k2];
+        t.x += v.x * wc[m] + v.y * ws[m];
+        t.y += v.y * wc[m] - v.x * ws[m];
+      }
+      T2[x] = t;
     }
-    Fr[x] = t;
+    __syncthreads();
+    R2* Fr = F + (size_t)(row0 + k) * PP;
+    for (int x = tid; x < PP; x += 256) {
+      const int k0 = x / (P * P), r = x % (P * P);
+      R2 t = c2<R>(0, 0);
+#pragma unroll
+      for (int j = 0; j < L; j++) {
+        const int m = (j * k0) % P;
+        const R2 v = T2[j * P * P + r];
+        t.x += v.x * wc[m] + v.y * ws[m];
+        t.y += v.y * wc[m] - v.x * ws[m];
+      }
+      Fr[x] = t;
+    }
   }
 }
 
@@ -1727,7 +1741,8 @@ static int runTyped(Exec& X, const double2* qin, double2* pout, int parts, int n
   const bool stored = X.m2lMode == DEFMM_M2L_STORED;
   if (far && X.nm2f && stored)
     for (int r = 0; r < nrhs; r++)
-      hipLaunchKernelGGL((k_m2f<R, L>), itemGrid(X.nm2f), dim3(256), 0, s, X.m2fSlot,
+      hipLaunchKernelGGL((k_m2f<R, L>), itemGrid((X.nm2f + kM2FRows - 1) / kM2FRows), dim3(256), 0, s,
+                         X.m2fSlot,
                          (const R2*)Mr(r), Fr(r), oc.twc, oc.tws, X.nm2f);
   mark(4);
   if (far && X.nGroups) {
@@ -1747,7 +1762,8 @@ static int runTyped(Exec& X, const double2* qin, double2* pout, int parts, int n
         const int g0 = X.hBatchGroupBeg[b], g1 = X.hBatchGroupBeg[b + 1];
         const int w0 = X.hBatchRowBeg[b], w1 = X.hBatchRowBeg[b + 1];
         for (int r = 0; r < nrhs; r++)
-          hipLaunchKernelGGL((k_m2f<R, L>), itemGrid(w1 - w0), dim3(256), 0, s, X.bRows + w0,
+          hipLaunchKernelGGL((k_m2f<R, L>), itemGrid((w1 - w0 + kM2FRows - 1) / kM2FRows), dim3(256),
+                             0, s, X.bRows + w0,
                              (const R2*)Mr(r), Fr(r), oc.twc, oc.tws, w1 - w0);
         for (int r0 = 0; r0 < nrhs;) r0 += m2lChunk<R, L>(X, s, g0, g1 - g0, r0, nrhs - r0, oc);
       }
