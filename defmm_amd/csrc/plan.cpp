@@ -10,11 +10,16 @@
 #include "plan.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <numeric>
 #include <tuple>
 #include <unordered_map>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 namespace dfm {
 
@@ -232,9 +237,16 @@ struct PairLists {
   std::vector<int> pt, ps;
 };
 
+// A walker owns its table-id map and caches; several walkers run on
+// disjoint cell pairs in parallel (their table ids are local and are renumbered
+// afterwards).  Marks are shared and set with atomic stores.
 struct Walker {
   Plan& pl;
-  PairLists& out;
+  PairLists* out;
+  int64_t nHF = 0, nLF = 0, nP2P = 0, pairsP2P = 0;
+  // task mode: pairs at level taskLevel are not walked but recorded as tasks
+  int taskLevel = -1;
+  std::vector<std::array<int, 2>>* tasks = nullptr;
   // table ids keyed by (level, fundamental offset)
   std::unordered_map<uint64_t, int> tabId;
   std::vector<std::array<int, 4>> tabKey;  // level, a, b, c
@@ -243,7 +255,7 @@ struct Walker {
   std::vector<std::vector<uint64_t>> win;
   std::vector<int> winW;
   int err = OK;
-  Walker(Plan& p, PairLists& o) : pl(p), out(o) {
+  Walker(Plan& p, PairLists* o) : pl(p), out(o) {
     // the offsets of M2L pairs at a level are bounded by the parents'
     // non-admissible range; a window of that half-width holds them all
     win.resize(pl.nlvl);
@@ -265,6 +277,9 @@ struct Walker {
     }
   }
 
+  static void mark(Tree& t, int c, int cn) {
+    __atomic_store_n(&t.marks[t.mark0[c] + cn], (uint8_t)1, __ATOMIC_RELAXED);
+  }
   static uint64_t key4(int l, int a, int b, int c) {
     return ((uint64_t)l << 48) | ((uint64_t)(uint32_t)a << 32) | ((uint64_t)(uint32_t)b << 16) |
            (uint64_t)(uint32_t)c;
@@ -310,14 +325,14 @@ struct Walker {
         uint64_t v = *slotw;
         int id = (int)(v & 0xFFFFFFFFu), hh = (int)((v >> 32) & 63), cn = (int)(v >> 40) - 1;
         if (hf) {
-          pl.T.marks[pl.T.mark0[i] + cn] = 1;
-          pl.S.marks[pl.S.mark0[j] + cn] = 1;
-          pl.st.hf_m2l++;
+          mark(pl.T, i, cn);
+          mark(pl.S, j, cn);
+          nHF++;
         } else {
-          pl.st.lf_m2l++;
+          nLF++;
         }
-        out.mt.push_back(i); out.ms.push_back(j); out.mtab.push_back(id);
-        out.mh.push_back(hh); out.mcone.push_back(cn);
+        out->mt.push_back(i); out->ms.push_back(j); out->mtab.push_back(id);
+        out->mh.push_back(hh); out->mcone.push_back(cn);
         return;
       }
     }
@@ -339,14 +354,14 @@ struct Walker {
         cn = bestCone(pl.dirs[pl.HFStage[lvl]], fZ);
         cone[ck] = cn;
       } else cn = it->second;
-      pl.T.marks[pl.T.mark0[i] + cn] = 1;
-      pl.S.marks[pl.S.mark0[j] + cn] = 1;
-      pl.st.hf_m2l++;
+      mark(pl.T, i, cn);
+      mark(pl.S, j, cn);
+      nHF++;
     } else {
-      pl.st.lf_m2l++;
+      nLF++;
     }
-    out.mt.push_back(i); out.ms.push_back(j); out.mtab.push_back(id);
-    out.mh.push_back(h); out.mcone.push_back(cn);
+    out->mt.push_back(i); out->ms.push_back(j); out->mtab.push_back(id);
+    out->mh.push_back(h); out->mcone.push_back(cn);
     if (slotw) *slotw = (uint64_t)(uint32_t)id | ((uint64_t)h << 32) | ((uint64_t)(cn + 1) << 40);
   }
   // hrzpass.hpp:61-91 (and the setup walk hrzpass.hpp:111-138, which visits
@@ -355,6 +370,10 @@ struct Walker {
     if (err) return;
     const Tree &T = pl.T, &S = pl.S;
     int l = T.lvl[i];
+    if (tasks && l == taskLevel) {
+      tasks->push_back({i, j});
+      return;
+    }
     if (pl.precomp[l] && l == S.lvl[j]) {
       if (l && pl.isHF[l]) {
         if (macHF(i, j)) { m2l(i, j, true); return; }
@@ -364,10 +383,10 @@ struct Walker {
       }
     }
     if (T.leaf(i) || S.leaf(j)) {
-      out.pt.push_back(i);
-      out.ps.push_back(j);
-      pl.st.p2p_calls++;
-      pl.st.p2p_pairs += (int64_t)T.nprt[i] * S.nprt[j];
+      out->pt.push_back(i);
+      out->ps.push_back(j);
+      nP2P++;
+      pairsP2P += (int64_t)T.nprt[i] * S.nprt[j];
       return;
     }
     for (int a = 0; a < T.nson[i]; a++)
@@ -459,6 +478,12 @@ int buildPlan(Plan& pl, const double* src, int64_t ns, const double* trg, int64_
   pl.boxRad = std::max(r1, r2);
   if (!(pl.boxRad > 0.)) { msg = "all points coincide"; return ERR_ARG; }
 
+  auto tick = [t0 = std::chrono::steady_clock::now()]() mutable {
+    auto t1 = std::chrono::steady_clock::now();
+    double ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    t0 = t1;
+    return (int64_t)ms;
+  };
   int e = buildTree(pl.T, trg, nt, pl.boxCtr, pl.boxRad, ncrit, lvmax);
   if (!e) e = buildTree(pl.S, src, ns, pl.boxCtr, pl.boxRad, ncrit, lvmax);
   if (e) { msg = "octree deeper than 30 levels (too many coincident points for Ncrit)"; return e; }
@@ -501,11 +526,99 @@ int buildPlan(Plan& pl, const double* src, int64_t ns, const double* trg, int64_
     t->marks.assign(tot, 0);
   }
 
+  pl.st.ms_tree = tick();
   // dual walk
   PairLists pr;
-  Walker w(pl, pr);
-  w.walk(0, 0);
-  if (w.err) { msg = "M2L offset outside the table range"; return w.err; }
+  std::vector<std::array<int, 4>> tabKeys;  // (level, fundamental offset) by table id
+  {
+    // The walk from the roots is split at a level with enough cell pairs:
+    // a serial walk records the pairs reached at that level as tasks (and
+    // lists the pairs decided above it); the tasks are walked in parallel,
+    // each into its own lists, and all lists are joined in serial-walk
+    // order, so the result does not depend on the thread count.
+    int taskLevel = std::min(3, nlvl - 1);
+    std::vector<std::array<int, 2>> tasks;
+    std::vector<PairLists> head(1);
+    Walker top(pl, &head[0]);
+    top.tasks = &tasks;
+    top.taskLevel = taskLevel;
+    // The pairs decided above the task level involve cells above it only,
+    // and M2L pairs are same-level, so every target keeps its serial-walk
+    // order when the top walk's lists come first and the tasks follow in the
+    // order met.
+    top.walk(0, 0);
+    if (top.err) { msg = "M2L offset outside the table range"; return top.err; }
+    const int64_t ntask = (int64_t)tasks.size();
+    std::vector<PairLists> tl(ntask);
+    std::vector<int> owner(ntask, 0);
+    int nthr = 1;
+#ifdef _OPENMP
+    nthr = omp_get_max_threads();
+#endif
+    std::vector<std::unique_ptr<Walker>> ws;
+    for (int t = 0; t < nthr; t++) ws.emplace_back(new Walker(pl, nullptr));
+    int werr = OK;
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : werr)
+    for (int64_t k = 0; k < ntask; k++) {
+      int tid = 0;
+#ifdef _OPENMP
+      tid = omp_get_thread_num();
+#endif
+      Walker& w = *ws[tid];
+      w.out = &tl[k];
+      w.walk(tasks[k][0], tasks[k][1]);
+      owner[k] = tid;
+      werr |= w.err;
+    }
+    if (werr) { msg = "M2L offset outside the table range"; return ERR_TABLE; }
+    // table ids: sorted (level, offset) keys, independent of thread timing
+    std::vector<std::array<int, 4>> all = top.tabKey;
+    for (auto& w : ws) all.insert(all.end(), w->tabKey.begin(), w->tabKey.end());
+    std::sort(all.begin(), all.end());
+    all.erase(std::unique(all.begin(), all.end()), all.end());
+    tabKeys = all;
+    auto remap = [&](const Walker& w) {
+      std::vector<int> m(w.tabKey.size());
+      for (size_t k = 0; k < m.size(); k++)
+        m[k] = (int)(std::lower_bound(all.begin(), all.end(), w.tabKey[k]) - all.begin());
+      return m;
+    };
+    std::vector<int> topMap = remap(top);
+    std::vector<std::vector<int>> wMap;
+    for (auto& w : ws) wMap.push_back(remap(*w));
+    // join at prefix-summed offsets, in parallel: head lists, then tasks
+    const int64_t nseg = ntask + 1;
+    auto seg = [&](int64_t k) -> const PairLists& { return k == 0 ? head[0] : tl[k - 1]; };
+    auto segMap = [&](int64_t k) -> const std::vector<int>& {
+      return k == 0 ? topMap : wMap[owner[k - 1]];
+    };
+    std::vector<int64_t> mOff(nseg + 1, 0), pOff(nseg + 1, 0);
+    for (int64_t k = 0; k < nseg; k++) {
+      mOff[k + 1] = mOff[k] + (int64_t)seg(k).mt.size();
+      pOff[k + 1] = pOff[k] + (int64_t)seg(k).pt.size();
+    }
+    for (auto* v : {&pr.mt, &pr.ms, &pr.mtab, &pr.mh, &pr.mcone}) v->resize(mOff[nseg]);
+    for (auto* v : {&pr.pt, &pr.ps}) v->resize(pOff[nseg]);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t k = 0; k < nseg; k++) {
+      const PairLists& s = seg(k);
+      const std::vector<int>& m = segMap(k);
+      const int64_t o = mOff[k], q = pOff[k];
+      for (size_t e = 0; e < s.mt.size(); e++) {
+        pr.mt[o + e] = s.mt[e];
+        pr.ms[o + e] = s.ms[e];
+        pr.mtab[o + e] = m[s.mtab[e]];
+        pr.mh[o + e] = s.mh[e];
+        pr.mcone[o + e] = s.mcone[e];
+      }
+      std::copy(s.pt.begin(), s.pt.end(), pr.pt.begin() + q);
+      std::copy(s.ps.begin(), s.ps.end(), pr.ps.begin() + q);
+    }
+    std::vector<PairLists>().swap(tl);
+    int64_t nHF = top.nHF, nLF = top.nLF, nP = top.nP2P, pP = top.pairsP2P;
+    for (auto& w : ws) { nHF += w->nHF; nLF += w->nLF; nP += w->nP2P; pP += w->pairsP2P; }
+    pl.st.hf_m2l = nHF; pl.st.lf_m2l = nLF; pl.st.p2p_calls = nP; pl.st.p2p_pairs = pP;
+  }
   if (!pl.noHF) { propagate(pl, pl.S); propagate(pl, pl.T); }
   numberSlots(pl, pl.S);
   numberSlots(pl, pl.T);
@@ -514,6 +627,7 @@ int buildPlan(Plan& pl, const double* src, int64_t ns, const double* trg, int64_
   pl.st.nlvl = nlvl; pl.st.cpt = cpt; pl.st.max_stage = ms;
   pl.st.ncells_t = pl.T.ncells(); pl.st.ncells_s = pl.S.ncells();
 
+  pl.st.ms_walk = tick();
   // ---- tables ----
   pl.embed.resize(LL);
   for (int i = 0; i < LL; i++) {
@@ -544,17 +658,17 @@ int buildPlan(Plan& pl, const double* src, int64_t ns, const double* trg, int64_
       tabZ[3 * i + k] = gridPoint(ip, L, 1.) - gridPoint(jn, L, 1.);
     }
   }
-  pl.ntab = (int)w.tabKey.size();
+  pl.ntab = (int)tabKeys.size();
   pl.m2lTab.assign((size_t)pl.ntab * PP, 0.);
 #pragma omp parallel
   {
     std::vector<cplx> g(PP);
 #pragma omp for schedule(dynamic, 4)
     for (int id = 0; id < pl.ntab; id++) {
-      int l = w.tabKey[id][0];
+      int l = tabKeys[id][0];
       double rad = pl.boxRad / double(1 << l);
       double v[3];
-      for (int k = 0; k < 3; k++) v[k] = 2. * double(w.tabKey[id][1 + k]) * rad;
+      for (int k = 0; k < 3; k++) v[k] = 2. * double(tabKeys[id][1 + k]) * rad;
       for (int i = 0; i < PP; i++) {
         double u[3];
         for (int k = 0; k < 3; k++) u[k] = v[k] + tabZ[3 * i + k] * rad;
@@ -565,7 +679,7 @@ int buildPlan(Plan& pl, const double* src, int64_t ns, const double* trg, int64_
     }
   }
   for (int id = 0; id < pl.ntab; id++) {
-    if (pl.isHF[w.tabKey[id][0]]) pl.st.hf_tables++; else pl.st.lf_tables++;
+    if (pl.isHF[tabKeys[id][0]]) pl.st.hf_tables++; else pl.st.lf_tables++;
   }
   // planewave.hpp:43-69: phase tables of every precomputation HF level
   pl.sigRow0.assign(nlvl, -1);
@@ -604,6 +718,7 @@ int buildPlan(Plan& pl, const double* src, int64_t ns, const double* trg, int64_
       }
   }
 
+  pl.st.ms_tables = tick();
   // ---- P2M / L2P leaves (upwpass.hpp:38-45, dwnpass.hpp:38-45) ----
   for (int c = 0; c < pl.S.ncells(); c++)
     if (pl.S.leaf(c) && pl.precomp[pl.S.lvl[c]]) {
@@ -625,15 +740,46 @@ int buildPlan(Plan& pl, const double* src, int64_t ns, const double* trg, int64_
     for (int k = 0; k < 3; k++) dv += d[k] * v[k];
     return std::exp(sgn * I1 * kappa * dv);
   };
+  // Lists are built per level in two passes over the level's cells (count,
+  // then fill at the prefix-summed offsets, in parallel): the same order as a
+  // serial walk over cells, slots and children.
+  auto cellsAt = [](const Tree& t, int l) {
+    std::vector<int> cs;
+    for (int c = 0; c < t.ncells(); c++)
+      if (t.lvl[c] == l && !t.leaf(c)) cs.push_back(c);
+    return cs;
+  };
+  auto sizeLevel = [](TransLevel& tl, int64_t nOut, int64_t nEnt) {
+    tl.out.resize(nOut);
+    tl.beg.resize(nOut + 1);
+    tl.beg[0] = 0;
+    tl.in.resize(nEnt);
+    tl.sgn.resize(nEnt);
+    tl.edv.resize(nEnt);
+    tl.sigIdx.resize(nEnt);
+  };
   for (int l = nlvl - 2; l >= 0; l--) {
     if (!pl.precomp[l]) continue;
     TransLevel tl;
     tl.level = l;
     tl.hf = pl.isHF[l] ? 1 : 0;
-    tl.beg.push_back(0);
     const Tree& S = pl.S;
-    for (int c = 0; c < S.ncells(); c++) {
-      if (S.lvl[c] != l || S.leaf(c)) continue;
+    const std::vector<int> cs = cellsAt(S, l);
+    const int64_t nc = (int64_t)cs.size();
+    std::vector<int64_t> oOff(nc + 1, 0), eOff(nc + 1, 0);
+    for (int64_t i = 0; i < nc; i++) {
+      const int c = cs[i];
+      oOff[i + 1] = oOff[i] + S.nslot[c];
+      eOff[i + 1] = eOff[i] + (int64_t)S.nslot[c] * S.nson[c];
+      if (pl.isHF[l]) { if (pl.HFStage[l]) pl.st.hf_m2m += S.nson[c]; else pl.st.lf2hf_m2m += S.nson[c]; }
+      else pl.st.lf_m2m += S.nson[c];
+    }
+    sizeLevel(tl, oOff[nc], eOff[nc]);
+    int missing = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : missing)
+    for (int64_t i = 0; i < nc; i++) {
+      const int c = cs[i];
+      int64_t o = oOff[i], e = eOff[i];
       for (int k = 0; k < S.nslot[c]; k++) {
         int os = S.slot0[c] + k;
         int d = S.slotDir[os];
@@ -649,17 +795,18 @@ int buildPlan(Plan& pl, const double* src, int64_t ns, const double* trg, int64_
           } else {
             in = S.slotOf(ch, -1);
           }
-          if (in < 0) { msg = "internal: missing child slot in M2M"; return ERR_STATE; }
+          if (in < 0) missing++;
           int sg = (v[0] < 0. ? 0 : 1) | (v[1] < 0. ? 0 : 2) | (v[2] < 0. ? 0 : 4);
-          tl.in.push_back(in); tl.sgn.push_back(sg); tl.edv.push_back(edv);
-          tl.sigIdx.push_back(tl.hf ? pl.sigRow0[l] + d : -1);
+          tl.in[e] = in; tl.sgn[e] = sg; tl.edv[e] = edv;
+          tl.sigIdx[e] = tl.hf ? pl.sigRow0[l] + d : -1;
+          e++;
         }
-        tl.out.push_back(os);
-        tl.beg.push_back((int)tl.in.size());
+        tl.out[o] = os;
+        tl.beg[o + 1] = (int)e;
+        o++;
       }
-      if (pl.isHF[l]) { if (pl.HFStage[l]) pl.st.hf_m2m += S.nson[c]; else pl.st.lf2hf_m2m += S.nson[c]; }
-      else pl.st.lf_m2m += S.nson[c];
     }
+    if (missing) { msg = "internal: missing child slot in M2M"; return ERR_STATE; }
     if (!tl.out.empty()) pl.m2m.push_back(std::move(tl));
   }
 
@@ -672,10 +819,47 @@ int buildPlan(Plan& pl, const double* src, int64_t ns, const double* trg, int64_
     TransLevel tl;
     tl.level = l;
     tl.hf = pl.isHF[l] ? 1 : 0;
-    tl.beg.push_back(0);
     const Tree& T = pl.T;
-    for (int c = 0; c < T.ncells(); c++) {
-      if (T.lvl[c] != l || T.leaf(c)) continue;
+    const bool byQuad = tl.hf && pl.HFStage[l];  // parent slots 4cd..4cd+3 only
+    const std::vector<int> cs = cellsAt(T, l);
+    const int64_t nc = (int64_t)cs.size();
+    std::vector<int64_t> oOff(nc + 1, 0), eOff(nc + 1, 0);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = 0; i < nc; i++) {
+      const int c = cs[i];
+      int64_t no = 0, ne = 0;
+      for (int a = 0; a < T.nson[c]; a++) {
+        int ch = T.sons[T.son0[c] + a];
+        no += T.nslot[ch];
+        for (int k = 0; k < T.nslot[ch]; k++) {
+          if (!byQuad) { ne += T.nslot[c]; continue; }
+          int cd = T.slotDir[T.slot0[ch] + k];
+          for (int pd = 4 * cd; pd < 4 * cd + 4; pd++) ne += T.slotOf(c, pd) >= 0;
+        }
+      }
+      oOff[i + 1] = no;
+      eOff[i + 1] = ne;
+    }
+    for (int64_t i = 0; i < nc; i++) {
+      oOff[i + 1] += oOff[i];
+      eOff[i + 1] += eOff[i];
+      const int c = cs[i];
+      if (pl.isHF[l]) { if (pl.HFStage[l]) pl.st.hf_l2l += T.nson[c]; else pl.st.hf2lf_l2l += T.nson[c]; }
+      else pl.st.lf_l2l += T.nson[c];
+    }
+    sizeLevel(tl, oOff[nc], eOff[nc]);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = 0; i < nc; i++) {
+      const int c = cs[i];
+      int64_t o = oOff[i], e = eOff[i];
+      auto entry = [&](int is, const double v[3], int sg) {
+        int pd = T.slotDir[is];
+        cplx edv = 1.;
+        if (tl.hf) edv = phase(pl.dirs[pl.HFStage[l]][pd].data(), v, 1.);
+        tl.in[e] = is; tl.sgn[e] = sg; tl.edv[e] = edv;
+        tl.sigIdx[e] = tl.hf ? pl.sigRow0[l] + pd : -1;
+        e++;
+      };
       for (int a = 0; a < T.nson[c]; a++) {
         int ch = T.sons[T.son0[c] + a];
         double v[3];
@@ -684,25 +868,25 @@ int buildPlan(Plan& pl, const double* src, int64_t ns, const double* trg, int64_
         for (int k = 0; k < T.nslot[ch]; k++) {
           int os = T.slot0[ch] + k;
           int cd = T.slotDir[os];
-          for (int pk = 0; pk < T.nslot[c]; pk++) {
-            int is = T.slot0[c] + pk;
-            int pd = T.slotDir[is];
-            if (tl.hf && pl.HFStage[l] && pd / 4 != cd) continue;
-            cplx edv = 1.;
-            if (tl.hf) edv = phase(pl.dirs[pl.HFStage[l]][pd].data(), v, 1.);
-            tl.in.push_back(is); tl.sgn.push_back(sg); tl.edv.push_back(edv);
-            tl.sigIdx.push_back(tl.hf ? pl.sigRow0[l] + pd : -1);
+          if (byQuad) {
+            // a cell's slots have ascending directions: same order as a scan
+            for (int pd = 4 * cd; pd < 4 * cd + 4; pd++) {
+              int is = T.slotOf(c, pd);
+              if (is >= 0) entry(is, v, sg);
+            }
+          } else {
+            for (int pk = 0; pk < T.nslot[c]; pk++) entry(T.slot0[c] + pk, v, sg);
           }
-          tl.out.push_back(os);
-          tl.beg.push_back((int)tl.in.size());
+          tl.out[o] = os;
+          tl.beg[o + 1] = (int)e;
+          o++;
         }
-        if (pl.isHF[l]) { if (pl.HFStage[l]) pl.st.hf_l2l++; else pl.st.hf2lf_l2l++; }
-        else pl.st.lf_l2l++;
       }
     }
     if (!tl.out.empty()) pl.l2l.push_back(std::move(tl));
   }
 
+  pl.st.ms_trans = tick();
   // ---- M2L, CSR by target slot; M2F only for slots read by M2L ----
   int64_t nm = (int64_t)pr.mt.size();
   std::vector<int> tslot(nm), sslot(nm);
@@ -715,22 +899,56 @@ int buildPlan(Plan& pl, const double* src, int64_t ns, const double* trg, int64_
     if (tslot[e2] < 0 || sslot[e2] < 0) missing++;
   }
   if (missing) { msg = "internal: missing M2L slot"; return ERR_STATE; }
-  pl.m2fIndex.assign(pl.S.nslots, -1);
-  for (size_t e2 = 0; e2 < nm; e2++)
-    if (pl.m2fIndex[sslot[e2]] < 0) pl.m2fIndex[sslot[e2]] = 0;
-  for (int s = 0; s < pl.S.nslots; s++)
-    if (pl.m2fIndex[s] == 0) { pl.m2fIndex[s] = (int)pl.m2fSlot.size(); pl.m2fSlot.push_back(s); }
+  {
+    std::vector<uint8_t> used(pl.S.nslots, 0);
+#pragma omp parallel for schedule(static)
+    for (int64_t e2 = 0; e2 < nm; e2++) __atomic_store_n(&used[sslot[e2]], (uint8_t)1, __ATOMIC_RELAXED);
+    pl.m2fIndex.assign(pl.S.nslots, -1);
+    for (int s = 0; s < pl.S.nslots; s++)
+      if (used[s]) { pl.m2fIndex[s] = (int)pl.m2fSlot.size(); pl.m2fSlot.push_back(s); }
+  }
+  // CSR by target slot, stable (entries of a target in walk order): a
+  // counting sort over contiguous chunks of the entries, one chunk per thread
   pl.m2lBeg.assign(pl.T.nslots + 1, 0);
-  for (size_t e2 = 0; e2 < nm; e2++) pl.m2lBeg[tslot[e2] + 1]++;
-  for (int s = 0; s < pl.T.nslots; s++) pl.m2lBeg[s + 1] += pl.m2lBeg[s];
   pl.m2lSrcF.resize(nm); pl.m2lTabId.resize(nm); pl.m2lHash.resize(nm);
   {
-    std::vector<int> fill(pl.m2lBeg.begin(), pl.m2lBeg.end() - 1);
-    for (size_t e2 = 0; e2 < nm; e2++) {
-      int pos = fill[tslot[e2]]++;
-      pl.m2lSrcF[pos] = pl.m2fIndex[sslot[e2]];
-      pl.m2lTabId[pos] = pr.mtab[e2];
-      pl.m2lHash[pos] = pr.mh[e2];
+    int nth = 1;
+#ifdef _OPENMP
+    nth = omp_get_max_threads();
+#endif
+    const int nts = pl.T.nslots;
+    std::vector<std::vector<int>> cnt(nth);
+#pragma omp parallel num_threads(nth)
+    {
+      int t = 0;
+#ifdef _OPENMP
+      t = omp_get_thread_num();
+#endif
+      const int64_t a = nm * t / nth, b = nm * (t + 1) / nth;
+      std::vector<int>& c = cnt[t];
+      c.assign(nts, 0);
+      for (int64_t e2 = a; e2 < b; e2++) c[tslot[e2]]++;
+#pragma omp barrier
+#pragma omp for schedule(static)
+      for (int s = 0; s < nts; s++) {
+        int run = 0;
+        for (int u = 0; u < nth; u++) {
+          int v = cnt[u][s];
+          cnt[u][s] = run;  // offset of chunk u within slot s
+          run += v;
+        }
+        pl.m2lBeg[s + 1] = run;
+      }
+#pragma omp single
+      for (int s = 0; s < nts; s++) pl.m2lBeg[s + 1] += pl.m2lBeg[s];
+      // implicit barrier after single
+      for (int64_t e2 = a; e2 < b; e2++) {
+        const int s = tslot[e2];
+        const int64_t pos = (int64_t)pl.m2lBeg[s] + c[s]++;
+        pl.m2lSrcF[pos] = pl.m2fIndex[sslot[e2]];
+        pl.m2lTabId[pos] = pr.mtab[e2];
+        pl.m2lHash[pos] = pr.mh[e2];
+      }
     }
   }
   pl.st.m2f_slots = (int64_t)pl.m2fSlot.size();
@@ -761,53 +979,98 @@ int buildPlan(Plan& pl, const double* src, int64_t ns, const double* trg, int64_
   {
     const int G = m2lGroupSize(L);
     pl.m2lG = G;
-    pl.m2lGroupBeg.push_back(0);
-    std::vector<std::array<int, 3>> tmp;  // srcF, target in group, tabHash
-    size_t i = 0;
-    while (i < pl.m2lOrder.size()) {
-      int t0 = pl.m2lOrder[i];
-      int l0 = pl.T.lvl[pl.T.slotCell[t0]], d0 = pl.T.slotDir[t0];
-      int ng = 0;
-      int tg[kM2LGroupMax];
-      while (ng < G && i < pl.m2lOrder.size()) {
-        int t = pl.m2lOrder[i];
-        if (pl.T.lvl[pl.T.slotCell[t]] != l0 || pl.T.slotDir[t] != d0) break;
-        tg[ng++] = t;
-        i++;
-      }
-      tmp.clear();
-      for (int g = 0; g < ng; g++)
-        for (int e = pl.m2lBeg[tg[g]]; e < pl.m2lBeg[tg[g] + 1]; e++)
-          tmp.push_back({pl.m2lSrcF[e], g, (pl.m2lTabId[e] << 6) | pl.m2lHash[e]});
-      std::stable_sort(tmp.begin(), tmp.end(),
-                       [](const std::array<int, 3>& a, const std::array<int, 3>& b) {
-                         return a[0] < b[0];
-                       });
-      for (int g = 0; g < G; g++) pl.m2lGroupTarget.push_back(g < ng ? tg[g] : -1);
-      size_t k = 0;
-      while (k < tmp.size()) {
-        int src = tmp[k][0];
-        int mask = 0;
-        int codes[kM2LGroupMax] = {0};
-        size_t k2 = k;
-        // a target reads a source row at most once per group entry; a repeat
-        // (the same pair listed twice) starts a new entry
-        while (k2 < tmp.size() && tmp[k2][0] == src && !((mask >> tmp[k2][1]) & 1)) {
-          mask |= 1 << tmp[k2][1];
-          codes[tmp[k2][1]] = tmp[k2][2];
-          k2++;
+    // group boundaries in the processing order (serial, cheap)
+    std::vector<int> gStart;  // index into m2lOrder of each group's first target
+    {
+      size_t i = 0;
+      while (i < pl.m2lOrder.size()) {
+        gStart.push_back((int)i);
+        int t0 = pl.m2lOrder[i];
+        int l0 = pl.T.lvl[pl.T.slotCell[t0]], d0 = pl.T.slotDir[t0];
+        int ng = 0;
+        while (ng < G && i < pl.m2lOrder.size()) {
+          int t = pl.m2lOrder[i];
+          if (pl.T.lvl[pl.T.slotCell[t]] != l0 || pl.T.slotDir[t] != d0) break;
+          ng++;
+          i++;
         }
-        pl.m2lGSrcF.push_back(src);
-        pl.m2lGMask.push_back(mask);
-        for (int g = 0; g < G; g++) pl.m2lGCode.push_back(codes[g]);
-        k = k2;
       }
-      pl.m2lGroupBeg.push_back((int)pl.m2lGSrcF.size());
+      gStart.push_back((int)pl.m2lOrder.size());
     }
-    pl.st.m2l_groups = (int64_t)pl.m2lGroupBeg.size() - 1;
+    const int64_t ngroups = (int64_t)gStart.size() - 1;
+    // entries of contiguous ranges of groups built in parallel, then joined
+    // in range order (the result does not depend on the thread count)
+    const int64_t nchunk = std::max<int64_t>(1, std::min<int64_t>(ngroups, 1024));
+    struct Part {
+      std::vector<int> src, mask, code, nent;
+    };
+    std::vector<Part> parts(nchunk);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t ch = 0; ch < nchunk; ch++) {
+      Part& pt = parts[ch];
+      std::vector<std::array<int, 3>> tmp;  // srcF, target in group, tabHash
+      const int64_t g0 = ngroups * ch / nchunk, g1 = ngroups * (ch + 1) / nchunk;
+      for (int64_t gi = g0; gi < g1; gi++) {
+        const int ng = gStart[gi + 1] - gStart[gi];
+        tmp.clear();
+        for (int g = 0; g < ng; g++) {
+          const int t = pl.m2lOrder[gStart[gi] + g];
+          for (int e = pl.m2lBeg[t]; e < pl.m2lBeg[t + 1]; e++)
+            tmp.push_back({pl.m2lSrcF[e], g, (pl.m2lTabId[e] << 6) | pl.m2lHash[e]});
+        }
+        std::stable_sort(tmp.begin(), tmp.end(),
+                         [](const std::array<int, 3>& a, const std::array<int, 3>& b) {
+                           return a[0] < b[0];
+                         });
+        int cnt = 0;
+        size_t k = 0;
+        while (k < tmp.size()) {
+          int src = tmp[k][0];
+          int mask = 0;
+          int codes[kM2LGroupMax] = {0};
+          size_t k2 = k;
+          // a target reads a source row at most once per group entry; a
+          // repeat (the same pair listed twice) starts a new entry
+          while (k2 < tmp.size() && tmp[k2][0] == src && !((mask >> tmp[k2][1]) & 1)) {
+            mask |= 1 << tmp[k2][1];
+            codes[tmp[k2][1]] = tmp[k2][2];
+            k2++;
+          }
+          pt.src.push_back(src);
+          pt.mask.push_back(mask);
+          for (int g = 0; g < G; g++) pt.code.push_back(codes[g]);
+          cnt++;
+          k = k2;
+        }
+        pt.nent.push_back(cnt);
+      }
+    }
+    int64_t total = 0;
+    for (const Part& pt : parts) total += (int64_t)pt.src.size();
+    if (total >= (int64_t)1 << 31) { msg = "M2L lists exceed 2^31 entries"; return ERR_MEMORY; }
+    pl.m2lGSrcF.reserve(total);
+    pl.m2lGMask.reserve(total);
+    pl.m2lGCode.reserve(total * G);
+    pl.m2lGroupBeg.reserve(ngroups + 1);
+    pl.m2lGroupTarget.reserve(ngroups * G);
+    pl.m2lGroupBeg.push_back(0);
+    for (int64_t gi = 0; gi < ngroups; gi++) {
+      const int ng = gStart[gi + 1] - gStart[gi];
+      for (int g = 0; g < G; g++) pl.m2lGroupTarget.push_back(g < ng ? pl.m2lOrder[gStart[gi] + g] : -1);
+    }
+    for (Part& pt : parts) {
+      pl.m2lGSrcF.insert(pl.m2lGSrcF.end(), pt.src.begin(), pt.src.end());
+      pl.m2lGMask.insert(pl.m2lGMask.end(), pt.mask.begin(), pt.mask.end());
+      pl.m2lGCode.insert(pl.m2lGCode.end(), pt.code.begin(), pt.code.end());
+      for (int c : pt.nent) pl.m2lGroupBeg.push_back(pl.m2lGroupBeg.back() + c);
+      Part().src.swap(pt.src);
+      Part().code.swap(pt.code);
+    }
+    pl.st.m2l_groups = ngroups;
     pl.st.m2l_group_entries = (int64_t)pl.m2lGSrcF.size();
   }
 
+  pl.st.ms_m2l = tick();
   // ---- P2P, CSR by target leaf (P2P.hpp:42-79 over whole cell ranges) ----
   // A pair with a non-leaf target cell is split over its descendant leaves;
   // each leaf's source ranges are sorted and touching ranges merged.

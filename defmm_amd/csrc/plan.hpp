@@ -66,6 +66,9 @@ struct Stats {
   int64_t p2p_items = 0, m2f_slots = 0, m2l_entries = 0, m2l_targets = 0;
   int64_t m2l_groups = 0, m2l_group_entries = 0;
   int64_t m2l_batches = 0, m2l_batch_rows = 0;  // batched M2L (set at upload)
+  // host set-up time per phase (ms): trees, dual walk, kernel tables and
+  // phase rows, M2M/L2L lists, M2L lists and groups, P2P lists
+  int64_t ms_tree = 0, ms_walk = 0, ms_tables = 0, ms_trans = 0, ms_m2l = 0, ms_p2p = 0;
 };
 
 // A per-level translation list (M2M or L2L): for every output slot, the
