@@ -109,7 +109,8 @@ def main():
         args.m2l = {v: k for k, v in modes.items()}[A.m2l_mode()]
     A.set_m2l(modes[args.m2l], batch_rows=args.batch_rows)
     st = A.stats()
-    log(f"setup {setup_s:.1f} s, device {A.device_bytes() / 1e9:.1f} GB, stats {st}")
+    setup_ms = {k[3:]: st[k] for k in st if k.startswith("ms_")}
+    log(f"setup {setup_s:.1f} s ({setup_ms} ms), device {A.device_bytes() / 1e9:.1f} GB, stats {st}")
 
     dev = torch.device("cuda:0")
     qd = torch.from_numpy(q).to(dev)
@@ -250,6 +251,7 @@ def main():
         "accuracy": {"rel_l2_err": e64, "rel_linf_err": einf64, "targets": args.samples,
                      "reference": "FP64 direct sum (GPU) on sampled targets"},
         "setup_s": setup_s,
+        "setup_ms_by_phase": setup_ms,
         "stage_ms": st64,
         "roofline": roof,
         "cpu_baseline": cpu,

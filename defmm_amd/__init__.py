@@ -35,7 +35,8 @@ STAT_NAMES = [
     "lf_tables", "hf_leaves", "lf_leaves", "hf_m2m", "lf2hf_m2m", "lf_m2m", "hf_l2p",
     "lf_l2p", "hf_l2l", "hf2lf_l2l", "lf_l2l", "nlvl", "cpt", "max_stage", "ncells_t",
     "ncells_s", "p2p_items", "m2f_slots", "m2l_entries", "m2l_targets", "m2l_groups",
-    "m2l_group_entries", "m2l_batches", "m2l_batch_rows",
+    "m2l_group_entries", "m2l_batches", "m2l_batch_rows", "ms_tree", "ms_walk", "ms_tables",
+    "ms_trans", "ms_m2l", "ms_p2p", "ms_upload",
 ]
 STAGE_NAMES = ["gather", "p2m", "m2m", "m2f", "m2l_f2l", "l2l", "l2p", "p2p", "scatter"]
 

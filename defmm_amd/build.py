@@ -34,7 +34,7 @@ def build(verbose=True):
           "-c", os.path.join(CSRC, "fmm.hip"), "-o", fmm_o])
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", plan_o, fmm_o, "-o", tmp,
-          "-lgomp"])
+          "-lgomp", "-lpthread"])
     os.replace(tmp, LIB)
     return LIB
 

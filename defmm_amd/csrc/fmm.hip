@@ -6,10 +6,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -1197,17 +1199,17 @@ struct Exec {
   bool typedPhase = false;  // allocations go to typedMem
 
   void freeTyped() {
-    for (void* m : typedMem) hipFree(m);
+    for (void* m : typedMem) (void)hipFree(m);
     typedMem.clear();
     bytes -= typedBytes;
     typedBytes = 0;
   }
   ~Exec() {
-    for (void* m : typedMem) hipFree(m);
-    for (void* m : mem) hipFree(m);
+    for (void* m : typedMem) (void)hipFree(m);
+    for (void* m : mem) (void)hipFree(m);
     for (auto& e : ev)
-      if (e) hipEventDestroy(e);
-    if (own) hipStreamDestroy(own);
+      if (e) (void)hipEventDestroy(e);
+    if (own) (void)hipStreamDestroy(own);
   }
 
   template <class X>
@@ -1345,21 +1347,31 @@ static int uploadTyped(Exec& X) {
     const size_t ntab = X.hTab.size() / (size_t)PP;
     std::vector<R2> rowsU(X.hRowKey.size() * (size_t)PP, c2h<R>(0., 0.));
     // (a plan without M2L keeps one zero row that is never read)
-    for (size_t k = 0; k < X.hRowKey.size() && ntab > 0; k++) {
-      const int key = X.hRowKey[k];
-      const int t = key / 48, v = (key / 8) % 6, z = key % 8;
-      if ((size_t)t >= ntab) {
+    for (int key : X.hRowKey)
+      if (ntab > 0 && (size_t)(key / 48) >= ntab) {
         g_err = "internal: M2L table id out of range";
         return ERR_STATE;
       }
-      for (int i = 0; i < PP; i++) {
-        int a[3] = {i / (P * P), (i / P) % P, i % P};
-        for (int j = 0; j < 3; j++)
-          if (((z >> j) & 1) && a[j]) a[j] = P - a[j];
-        const int src = (a[kVarSigma[v][0]] * P + a[kVarSigma[v][1]]) * P + a[kVarSigma[v][2]];
-        const cplx& c = X.hTab[t * PP + src];
-        rowsU[k * PP + i] = c2h<R>(c.real(), c.imag());
+    auto fillRows = [&](size_t k0, size_t k1) {
+      for (size_t k = k0; k < k1 && ntab > 0; k++) {
+        const int key = X.hRowKey[k];
+        const int t = key / 48, v = (key / 8) % 6, z = key % 8;
+        for (int i = 0; i < PP; i++) {
+          int a[3] = {i / (P * P), (i / P) % P, i % P};
+          for (int j = 0; j < 3; j++)
+            if (((z >> j) & 1) && a[j]) a[j] = P - a[j];
+          const int src = (a[kVarSigma[v][0]] * P + a[kVarSigma[v][1]]) * P + a[kVarSigma[v][2]];
+          const cplx& c = X.hTab[(size_t)t * PP + src];
+          rowsU[k * PP + i] = c2h<R>(c.real(), c.imag());
+        }
       }
+    };
+    {
+      const size_t nk = X.hRowKey.size();
+      const unsigned nth = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+      std::vector<std::thread> th;
+      for (unsigned t = 0; t < nth; t++) th.emplace_back(fillRows, nk * t / nth, nk * (t + 1) / nth);
+      for (auto& x : th) x.join();
     }
     CKR(X.upv(d, rowsU));
     X.U = d;
@@ -1715,7 +1727,8 @@ static int runTyped(Exec& X, const double2* qin, double2* pout, int parts, int n
   auto Fr = [&](int r) { return (R2*)X.F + r * X.fStride; };
   const R2* sig = (const R2*)X.sig;
   const R2* sig2 = (const R2*)X.sig2;
-  auto mark = [&](int k) { hipEventRecord(X.ev[k], s); };
+  // (a failed record surfaces at the launch check below or in stage_times)
+  auto mark = [&](int k) { (void)hipEventRecord(X.ev[k], s); };
   mark(0);
   hipLaunchKernelGGL(k_zero<R2>, dim3(nblk((int64_t)(nrhs * nt), 256)), dim3(256), 0, s,
                      (R2*)X.p, (int64_t)(nrhs * nt));
@@ -1890,8 +1903,16 @@ int defmm_plan_create(const double* src_xyz, int64_t n_src, const double* trg_xy
   }
   defmm_plan_s* h = new defmm_plan_s();
   h->x.prec = precision;
-  hipGetDevice(&h->x.device);
+  if (hipGetDevice(&h->x.device) != hipSuccess) {
+    g_err = "hipGetDevice failed";
+    delete h;
+    delete pl;
+    return DEFMM_EHIP;
+  }
+  const auto tu0 = std::chrono::steady_clock::now();
   rc = dfm::upload(h->x, *pl);
+  h->x.st.ms_upload =
+      (int64_t)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tu0).count();
   delete pl;
   if (rc == dfm::OK) {
     if (hipStreamCreate(&h->x.own) != hipSuccess) rc = DEFMM_EHIP;
@@ -1980,7 +2001,9 @@ int defmm_plan_stats(defmm_plan plan, int64_t* out, int n) {
                              s.lf_l2p, s.hf_l2l, s.hf2lf_l2l, s.lf_l2l, s.nlvl, s.cpt,
                              s.max_stage, s.ncells_t, s.ncells_s, s.p2p_items, s.m2f_slots,
                              s.m2l_entries, s.m2l_targets, s.m2l_groups,
-                             s.m2l_group_entries, s.m2l_batches, s.m2l_batch_rows};
+                             s.m2l_group_entries, s.m2l_batches, s.m2l_batch_rows,
+                             s.ms_tree, s.ms_walk, s.ms_tables, s.ms_trans, s.ms_m2l,
+                             s.ms_p2p, s.ms_upload};
   for (int i = 0; i < n && i < DEFMM_NSTATS; i++) out[i] = v[i];
   return DEFMM_OK;
 }
@@ -2089,7 +2112,7 @@ int defmm_direct(const double* src_xyz, int64_t n_src, const double* q, const do
   }
   if (!rc) ck(hipMemcpy(p, dp, n_trg * 16, hipMemcpyDeviceToHost), "hipMemcpy");
   for (void* m : {ds, dq, dt, dp})
-    if (m) hipFree(m);
+    if (m) (void)hipFree(m);
   return rc;
 }
 
@@ -2116,8 +2139,8 @@ int defmm_plan_build_stats(const double* src_xyz, int64_t n_src, const double* t
 
 int defmm_plan_destroy(defmm_plan plan) {
   if (!plan) return DEFMM_OK;
-  hipSetDevice(plan->x.device);
-  hipStreamSynchronize(plan->x.own);
+  (void)hipSetDevice(plan->x.device);
+  (void)hipStreamSynchronize(plan->x.own);
   delete plan;
   return DEFMM_OK;
 }

@@ -1143,6 +1143,7 @@ int buildPlan(Plan& pl, const double* src, int64_t ns, const double* trg, int64_
       pl.p2pBeg.push_back((int)pl.p2pSrcB.size());
     }
   }
+  pl.st.ms_p2p = tick();
   return validatePlan(pl, msg);
 }
 

@@ -69,6 +69,7 @@ struct Stats {
   // host set-up time per phase (ms): trees, dual walk, kernel tables and
   // phase rows, M2M/L2L lists, M2L lists and groups, P2P lists
   int64_t ms_tree = 0, ms_walk = 0, ms_tables = 0, ms_trans = 0, ms_m2l = 0, ms_p2p = 0;
+  int64_t ms_upload = 0;  // device upload (set by the executor)
 };
 
 // A per-level translation list (M2M or L2L): for every output slot, the

@@ -187,6 +187,10 @@ enum {
   DEFMM_STAT_M2F_SLOTS, DEFMM_STAT_M2L_ENTRIES, DEFMM_STAT_M2L_TARGETS,
   DEFMM_STAT_M2L_GROUPS, DEFMM_STAT_M2L_GROUP_ENTRIES, DEFMM_STAT_M2L_BATCHES,
   DEFMM_STAT_M2L_BATCH_ROWS,
+  /* set-up time in milliseconds: octrees, dual walk, kernel tables and phase
+     rows, M2M/L2L lists, M2L lists, P2P lists, device upload */
+  DEFMM_STAT_MS_TREE, DEFMM_STAT_MS_WALK, DEFMM_STAT_MS_TABLES, DEFMM_STAT_MS_TRANS,
+  DEFMM_STAT_MS_M2L, DEFMM_STAT_MS_P2P, DEFMM_STAT_MS_UPLOAD,
   DEFMM_NSTATS
 };
 
