@@ -94,6 +94,7 @@ def main():
     from defmm_amd import workloads as wl
 
     pts, kappa, L, ncrit = wl.make(args.workload, seed=1)
+    geo, _, nl, _, _ = wl.CONFIGS[args.workload]
     n = len(pts)
     q = wl.charges(n, seed=7)
     log(f"workload {args.workload}: N={n} kappa={kappa:.6f} L={L} Ncrit={ncrit}")
@@ -244,8 +245,8 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms64,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"{args.workload}: {n:,} points uniform on the unit sphere, "
-                               f"128 wavelengths, order {L}, leaf<=64, 1 RHS, FP64 drop-in",
+        "config": {"workload": f"{args.workload}: {n:,} points ({geo}), {nl:g} wavelengths, "
+                               f"order {L}, leaf<={ncrit - 1}, 1 RHS, FP64 drop-in",
                    "n_points": n, "kappa": kappa, "order": L, "ncrit": ncrit, "nrhs": 1,
                    "parallelism": "single GPU"},
         "accuracy": {"rel_l2_err": e64, "rel_linf_err": einf64, "targets": args.samples,

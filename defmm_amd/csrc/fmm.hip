@@ -402,7 +402,7 @@ struct M2LShape {
   static constexpr int NWpref = (PP + 255) / 256 < 3 ? (PP + 255) / 256 : 3;
   static constexpr int NW = NWmin > NWpref ? NWmin : NWpref;
   static constexpr int TB = ((PP + NW - 1) / NW + 63) / 64 * 64;
-  static constexpr int G = kM2LGroupMax / 2;
+  static constexpr int G = m2lGroupSize(L);
 };
 // With NR right-hand sides per launch a thread holds G x NW x NR
 // accumulators: fewest frequencies per thread (blocks of up to 1024 threads).
@@ -415,11 +415,12 @@ struct M2LShapeN {
 };
 // Right-hand sides per M2L launch: at most 16 (double) or 32 (float)
 // complex accumulators per thread.
+// (double at L = 8: one, the 2-wide kernel spills)
 template <typename R, int L>
 constexpr int m2lMaxNR() {
   constexpr int NW = (M2LShape<L>::PP + 1023) / 1024;
   constexpr int cap = (sizeof(R) == 8 ? 16 : 32) / (M2LShape<L>::G * NW);
-  return cap >= 8 ? 8 : cap >= 4 ? 4 : cap >= 2 ? 2 : 1;
+  return (sizeof(R) == 8 && L >= 8) ? 1 : cap >= 8 ? 8 : cap >= 4 ? 4 : cap >= 2 ? 2 : 1;
 }
 
 template <int PP>

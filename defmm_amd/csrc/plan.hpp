@@ -17,9 +17,10 @@ namespace dfm {
 typedef std::complex<double> cplx;
 
 // Target slots per M2L work group: a thread holds one accumulator per group
-// target and frequency it owns.
+// target and frequency it owns (2 targets at L = 8, where a thread owns 4
+// frequencies, so that the FP64 kernel fits its registers).
 constexpr int kM2LGroupMax = 8;
-inline int m2lGroupSize(int /*L*/) { return kM2LGroupMax / 2; }
+constexpr int m2lGroupSize(int L) { return L >= 8 ? 2 : 4; }
 
 // Error codes returned through the C ABI.
 enum {

@@ -120,6 +120,10 @@ CONFIGS = {
     # name: (geometry, n, n_lambda, L, Ncrit)  -- Ncrit = leaf size + 1
     "cube20k_32l": ("cube_edges", 20000, 32, 4, 65),
     "sphere1m_128l": ("sphere", 1000000, 128, 5, 65),
+    # same sphere at 32 wavelengths with order 7: the relative-error <= 1e-6 regime
+    "sphere1m_32l_L7": ("sphere", 1000000, 32, 7, 65),
+    "sphere1m_32l_L8": ("sphere", 1000000, 32, 8, 65),
+    "sphere1m_16l_L7": ("sphere", 1000000, 16, 7, 65),
 }
 
 
