@@ -53,6 +53,66 @@ def cpu_baseline(n=20000, L=5, ncrit=65):
                       f"({ta:.1f} s; setup {ts:.1f} s excluded)"}
 
 
+def gate_regime(name, steps, warmup, samples, nrhs):
+    """The same measurement, compact, on a workload where this method reaches
+    relative error <= 1e-6 (order 7 at 16 wavelengths): FP64, mixed, mixed
+    with nrhs right-hand sides; errors against the GPU FP64 direct sum."""
+    import torch
+    import defmm_amd as dm
+    from defmm_amd import workloads as wl
+    pts, kappa, L, ncrit = wl.make(name, seed=1)
+    geo, _, nl, _, _ = wl.CONFIGS[name]
+    n = len(pts)
+    Q = np.stack([wl.charges(n, seed=7)] + [wl.charges(n, seed=100 + r) for r in range(1, nrhs)])
+    A = dm.IBFMM_Mat(precision=dm.FP64)
+    A.setSourceParticles(pts)
+    A.setTargetParticles(pts)
+    t0 = time.time()
+    A.prcmpt(L, ncrit, kappa)
+    setup_s = time.time() - t0
+    idx = np.linspace(0, n - 1, samples).astype(np.int64)
+    exact = dm.direct(pts, Q[0], pts[idx], kappa)
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    Qd = torch.from_numpy(Q).to(dev)
+    Pd = torch.zeros((nrhs, n), dtype=torch.complex128, device=dev)
+
+    def timed(nr):
+        for _ in range(warmup):
+            A.apply_batch_device(Qd.data_ptr(), Pd.data_ptr(), nr, stream.cuda_stream)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            A.apply_batch_device(Qd.data_ptr(), Pd.data_ptr(), nr, stream.cuda_stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        p = Pd[0].cpu().numpy()[idx]
+        err = float(np.sqrt((np.abs(p - exact) ** 2).sum() / (np.abs(exact) ** 2).sum()))
+        return ms, err
+
+    ms64, e64 = timed(1)
+    A.set_precision(dm.MIXED)
+    msmx, emx = timed(1)
+    A.set_rhs(nrhs)
+    msb, eb = timed(nrhs)
+    A.destroy()
+    out = {"workload": f"{name}: {n:,} points ({geo}), {nl:g} wavelengths, order {L}, "
+                       f"leaf<={ncrit - 1}",
+           "setup_s": setup_s, "targets": samples,
+           "fp64": {"value": n / (ms64 * 1e-3) / 1e6, "ms_per_step": ms64, "rel_l2_err": e64},
+           "mixed": {"value": n / (msmx * 1e-3) / 1e6, "ms_per_step": msmx, "rel_l2_err": emx,
+                     "speedup_vs_fp64": ms64 / msmx},
+           "mixed_batched": {"value": n * nrhs / (msb * 1e-3) / 1e6, "nrhs": nrhs,
+                             "ms_per_step": msb, "rel_l2_err_first_rhs": eb,
+                             "speedup_vs_fp64": ms64 * nrhs / msb},
+           "unit": "Mpts·RHS/s"}
+    log(f"[gate {name}] fp64 {ms64:.2f} ms (err {e64:.3e}), mixed {msmx:.2f} ms (err {emx:.3e}), "
+        f"mixed x{nrhs} {msb:.2f} ms (err {eb:.3e})")
+    return out
+
+
 def read_traffic(kernel, workload):
     path = os.path.join(ROOT, "profiles", f"{kernel}_traffic_{workload}.txt")
     if not os.path.exists(path):
@@ -80,6 +140,8 @@ def main():
                     help="also time the other M2L variants (FP64)")
     ap.add_argument("--nrhs", type=int, default=4,
                     help="right-hand sides of the mixed batched pass (0 to skip)")
+    ap.add_argument("--gate-workload", default="sphere1m_16l_L7",
+                    help="workload of the relative-error <= 1e-6 measurement ('none' to skip)")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="after timing, copy this many GiB device-to-device (a kernel of known "
                          "traffic, to scale rocprof byte counters)")
@@ -234,6 +296,11 @@ def main():
         log(f"copied {nb * 16} bytes device-to-device")
         del src, dst
 
+    gate = None
+    if args.gate_workload != "none":
+        gate = gate_regime(args.gate_workload, args.steps, args.warmup, args.samples,
+                           max(2, args.nrhs) if args.nrhs else 4)
+
     cpu = None
     if not args.no_cpu:
         t0 = time.time()
@@ -258,6 +325,7 @@ def main():
         "cpu_baseline": cpu,
         "mixed": mixed,
         "mixed_batched": batched,
+        "accuracy_gate_1e-6": gate,
         "m2l_variants_fp64": alt,
         "plan": {k: st[k] for k in ("p2p_pairs", "hf_m2l", "src_slots", "trg_slots",
                                       "m2f_slots", "m2l_entries", "m2l_group_entries",
