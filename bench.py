@@ -138,8 +138,11 @@ def main():
                     help="workspace rows of the batched M2L (library default if unset)")
     ap.add_argument("--compare-m2l", action="store_true",
                     help="also time the other M2L variants (FP64)")
-    ap.add_argument("--nrhs", type=int, default=4,
-                    help="right-hand sides of the mixed batched pass (0 to skip)")
+    ap.add_argument("--nrhs", type=int, default=2,
+                    help="right-hand sides of the mixed batched pass (0 to skip); 2 keeps the "
+                         "stored-Fourier M2L at 1M points / 128 wavelengths")
+    ap.add_argument("--gate-nrhs", type=int, default=4,
+                    help="right-hand sides of the batched pass of the 1e-6 workload")
     ap.add_argument("--gate-workload", default="sphere1m_16l_L7",
                     help="workload of the relative-error <= 1e-6 measurement ('none' to skip)")
     ap.add_argument("--copy-gib", type=float, default=0.0,
@@ -299,7 +302,7 @@ def main():
     gate = None
     if args.gate_workload != "none":
         gate = gate_regime(args.gate_workload, args.steps, args.warmup, args.samples,
-                           max(2, args.nrhs) if args.nrhs else 4)
+                           args.gate_nrhs)
 
     cpu = None
     if not args.no_cpu:

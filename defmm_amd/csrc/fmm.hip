@@ -405,12 +405,14 @@ struct M2LShape {
   static constexpr int G = m2lGroupSize(L);
 };
 // With NR right-hand sides per launch a thread holds G x NW x NR
-// accumulators: fewest frequencies per thread (blocks of up to 1024 threads).
+// accumulators in the same shape (measured faster than one frequency per
+// thread in 768-thread blocks: 144 vs 187 ms per right-hand side at 1M
+// points / 128 wavelengths / L = 5, float, two right-hand sides).
 template <int L, int NR>
 struct M2LShapeN {
   static constexpr int PP = M2LShape<L>::PP;
-  static constexpr int NW = NR == 1 ? M2LShape<L>::NW : (PP + 1023) / 1024;
-  static constexpr int TB = NR == 1 ? M2LShape<L>::TB : ((PP + NW - 1) / NW + 63) / 64 * 64;
+  static constexpr int NW = M2LShape<L>::NW;
+  static constexpr int TB = M2LShape<L>::TB;
   static constexpr int G = M2LShape<L>::G;
 };
 // Right-hand sides per M2L launch: at most 16 (double) or 32 (float)
@@ -418,7 +420,7 @@ struct M2LShapeN {
 // (double at L = 8: one, the 2-wide kernel spills)
 template <typename R, int L>
 constexpr int m2lMaxNR() {
-  constexpr int NW = (M2LShape<L>::PP + 1023) / 1024;
+  constexpr int NW = M2LShape<L>::NW;
   constexpr int cap = (sizeof(R) == 8 ? 16 : 32) / (M2LShape<L>::G * NW);
   return (sizeof(R) == 8 && L >= 8) ? 1 : cap >= 8 ? 8 : cap >= 4 ? 4 : cap >= 2 ? 2 : 1;
 }
