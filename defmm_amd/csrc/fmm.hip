@@ -211,20 +211,30 @@ __global__ __launch_bounds__(128) void k_trans(
   R2 acc[NI];
 #pragma unroll
   for (int m = 0; m < NI; m++) acc[m] = c2<R>(0, 0);
+  // the input row (and its phase row) of entry e + 1 is loaded into
+  // registers while entry e is swept, hiding the load latency
+  R2 nv[NI], np[NI];
+  auto fetch = [&](int e) {
+    const int s = in[e], row = sigIdx[e];
+#pragma unroll
+    for (int m = 0; m < NI; m++) {
+      const int I = min(tid + 128 * m, LL - 1);
+      nv[m] = src[(size_t)s * LL + I];
+      np[m] = row >= 0 ? preTab[(size_t)row * LL + I] : c2<R>(1, 0);
+    }
+  };
+  if (e0 < e1) fetch(e0);
   for (int e = e0; e < e1; e++) {
-    const int s = in[e], row = sigIdx[e], sg = sgn[e];
+    const int row = sigIdx[e], sg = sgn[e];
     const double2 ed = edv[e];
     const R2 edr = c2<R>((R)ed.x, (R)ed.y);
     __syncthreads();
 #pragma unroll
     for (int m = 0; m < NI; m++) {
       const int I = tid + 128 * m;
-      if (I < LL) {
-        R2 v = src[(size_t)s * LL + I];
-        if (row >= 0) v = cmul(v, preTab[(size_t)row * LL + I]);
-        a[I] = v;
-      }
+      if (I < LL) a[I] = cmul(nv[m], np[m]);
     }
+    if (e + 1 < e1) fetch(e + 1);
     __syncthreads();
     {
       const R* M2 = mt[(sg >> 2) & 1];
