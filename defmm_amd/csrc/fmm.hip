@@ -911,7 +911,15 @@ template <>
 __device__ __forceinline__ void p2pKernel<float, true>(double xt, double yt, double zt,
                                                        double xs, double ys, double zs,
                                                        double kappa, float& co, float& si) {
-  p2pKernel<float, false>(xt, yt, zt, xs, ys, zs, kappa, co, si);
+  const double x = xt - xs, y = yt - ys, z = zt - zs;
+  const double R2v = x * x + y * y + z * z;
+  const float y0 = (R2v < 1.e-16) ? 0.f : rsqrtf((float)R2v);
+  const double r0 = R2v * (double)y0;
+  const double r = fma(0.5 * (double)y0, fma(-r0, r0, R2v), r0);
+  const float K = y0 * 0.079577471545947667884f;
+  sincosRedF(kappa * r, si, co);
+  co *= K;
+  si *= K;
 }
 
 // NR right-hand sides: charges q + r * qStride; the partial sums of item k,

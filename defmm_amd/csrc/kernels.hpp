@@ -41,10 +41,15 @@ __device__ __forceinline__ A cmulc(A a, B b) {
 }
 
 // exp(i*arg).  double: sincosRed (sincos_red.hpp).  float: the argument is
-// reduced modulo 2*pi in double first, so that a float evaluation keeps an
-// absolute phase error near 1e-7 for arguments of hundreds of radians.
+// reduced in double first (sincosRedF; beyond 2^20 rad modulo 2*pi), so that
+// a float evaluation keeps an absolute phase error near 1e-7 for arguments of
+// hundreds of radians.
 __device__ __forceinline__ void expi(double arg, double& c, double& s) { sincosRed(arg, s, c); }
 __device__ __forceinline__ void expi(double arg, float& c, float& s) {
+  if (fabs(arg) <= 1048576.0) {
+    sincosRedF(arg, s, c);      // reduction by pi/2 in double, float polynomials
+    return;
+  }
   const double inv2pi = 0.15915494309189533577;
   double t = arg * inv2pi;
   t -= rint(t);                 // t in [-1/2, 1/2] turns

@@ -71,6 +71,31 @@ DFM_HD inline void sincosRed(double x, double& s, double& c) {
   sincosRedSmall(x, s, c);
 }
 
+// Float sine and cosine of a double argument (the mixed-precision phases):
+// the same reduction by pi/2 in double, then float Taylor polynomials on
+// |y| <= pi/4 truncated after y^9/9! and y^8/8! (remainders below 2e-9 and
+// 3e-8, under the float rounding of 6e-8).  |x| <= 2^20.
+DFM_HD inline void sincosRedF(double x, float& s, float& c) {
+  const double n = rint(x * 0.6366197723675814);
+  double yd = fma(-n, 1.5707963267948966, x);
+  yd = fma(-n, 6.123233995736766e-17, yd);
+  const float y = (float)yd, z = y * y;
+  float ps = 2.7557319e-06f;               // 1/9!
+  ps = fmaf(ps, z, -0.0001984127f);        // -1/7!
+  ps = fmaf(ps, z, 0.008333334f);          // 1/5!
+  ps = fmaf(ps, z, -0.16666667f);          // -1/3!
+  const float sy = fmaf(ps * z, y, y);
+  float pc = 2.4801588e-05f;               // 1/8!
+  pc = fmaf(pc, z, -0.0013888889f);        // -1/6!
+  pc = fmaf(pc, z, 0.041666668f);          // 1/4!
+  pc = fmaf(pc, z, -0.5f);                 // -1/2!
+  const float cy = fmaf(pc, z, 1.0f);
+  const int q = (int)n & 3;
+  const float a = (q & 1) ? cy : sy, b = (q & 1) ? sy : cy;
+  s = (q & 2) ? -a : a;
+  c = ((q + 1) & 2) ? -b : b;
+}
+
 }  // namespace dfm
 
 #endif

@@ -4,6 +4,7 @@
 // tiny arguments.
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
 #include <random>
 
 #include "../../defmm_amd/csrc/sincos_red.hpp"
@@ -33,5 +34,16 @@ int main() {
   check(0.0);
   check(2.0e6);  // library branch
   printf("max_abs_err_sin %.3e max_abs_err_cos %.3e at %.17g\n", worst_s, worst_c, worst_x);
+  // float variant (arguments up to 2^20)
+  double wf = 0;
+  std::uniform_real_distribution<double> uf(-1.0e6, 1.0e6);
+  for (int i = 0; i < 4000000; i++) {
+    double x = i < 2000000 ? uf(g) : (i - 3000000) * 1e-3;
+    float s, c;
+    dfm::sincosRedF(x, s, c);
+    long double xs = sinl((long double)x), xc = cosl((long double)x);
+    wf = std::max(wf, (double)std::max(fabsl((long double)s - xs), fabsl((long double)c - xc)));
+  }
+  printf("max_abs_err_float %.3e\n", wf);
   return 0;
 }
