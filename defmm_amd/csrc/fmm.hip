@@ -81,7 +81,8 @@ __global__ void k_scatter(const typename C2T<R>::T* __restrict__ p, const int* _
 //   HF leaf, slot of direction d:
 //     M[I] = conj(sig2_d[I]) * sum_j S_I((y_j-c)/r) exp(-i kappa d.(y_j-c)) q_j
 //   LF leaf: M[I] = sum_j S_I((y_j-c)/r) q_j
-// One block per leaf; particles are staged 64 at a time in LDS.
+// One block per leaf; particles are staged 64 at a time in LDS, and 4 slots
+// (directions) are accumulated per pass over them, sharing the basis products.
 template <typename R, int L>
 __global__ __launch_bounds__(128) void k_p2m(const int* __restrict__ leaves, DevCells S,
                                              DevLevels lv, const double* __restrict__ px,
@@ -95,8 +96,9 @@ __global__ __launch_bounds__(128) void k_p2m(const int* __restrict__ leaves, Dev
   typedef typename C2T<R>::T R2;
   constexpr int LL = L * L * L;
   constexpr int NI = (LL + 127) / 128;
+  constexpr int C = 4;  // slots per pass: they share the basis products
   __shared__ R Sw[3][L][64];
-  __shared__ R2 w[64];
+  __shared__ R2 w[C][64];
   __shared__ double rel[3][64];
   const int item = itemIndex();
   if (item >= nItems) return;
@@ -107,46 +109,52 @@ __global__ __launch_bounds__(128) void k_p2m(const int* __restrict__ leaves, Dev
   const int doff = lv.dirOff[lvl], srow = lv.sigRow0[lvl];
   const bool single = n <= 64;
   const int tid = threadIdx.x;
-  for (int k = 0; k < ns; k++) {
-    const int d = hf ? S.slotDir[s0 + k] : 0;
-    double dx = 0., dy = 0., dz = 0.;
-    if (hf) {
-      dx = dirs[3 * (doff + d)];
-      dy = dirs[3 * (doff + d) + 1];
-      dz = dirs[3 * (doff + d) + 2];
-    }
-    R2 acc[NI];
+  for (int k0 = 0; k0 < ns; k0 += C) {
+    const int kc = min(C, ns - k0);
+    R2 acc[NI][C];
 #pragma unroll
-    for (int m = 0; m < NI; m++) acc[m] = c2<R>(0, 0);
+    for (int m = 0; m < NI; m++)
+#pragma unroll
+      for (int u = 0; u < C; u++) acc[m][u] = c2<R>(0, 0);
     for (int j0 = 0; j0 < n; j0 += 64) {
       const int cn = min(64, n - j0);
       __syncthreads();
-      if (tid < cn) {
+      if (tid < cn && (!single || k0 == 0)) {
         const int j = tid;
-        if (!single || k == 0) {
-          double r0 = px[pb + j0 + j] - cx, r1 = py[pb + j0 + j] - cy, r2 = pz[pb + j0 + j] - cz;
-          rel[0][j] = r0;
-          rel[1][j] = r1;
-          rel[2][j] = r2;
-          R t[LMAX];
-          lagrangeAll<R>((R)(r0 / rad), L, oc.xi, oc.iden, t);
+        double r0 = px[pb + j0 + j] - cx, r1 = py[pb + j0 + j] - cy, r2 = pz[pb + j0 + j] - cz;
+        rel[0][j] = r0;
+        rel[1][j] = r1;
+        rel[2][j] = r2;
+        R t[LMAX];
+        lagrangeAll<R>((R)(r0 / rad), L, oc.xi, oc.iden, t);
 #pragma unroll
-          for (int i = 0; i < L; i++) Sw[0][i][j] = t[i];
-          lagrangeAll<R>((R)(r1 / rad), L, oc.xi, oc.iden, t);
+        for (int i = 0; i < L; i++) Sw[0][i][j] = t[i];
+        lagrangeAll<R>((R)(r1 / rad), L, oc.xi, oc.iden, t);
 #pragma unroll
-          for (int i = 0; i < L; i++) Sw[1][i][j] = t[i];
-          lagrangeAll<R>((R)(r2 / rad), L, oc.xi, oc.iden, t);
+        for (int i = 0; i < L; i++) Sw[1][i][j] = t[i];
+        lagrangeAll<R>((R)(r2 / rad), L, oc.xi, oc.iden, t);
 #pragma unroll
-          for (int i = 0; i < L; i++) Sw[2][i][j] = t[i];
+        for (int i = 0; i < L; i++) Sw[2][i][j] = t[i];
+      }
+      __syncthreads();
+      // w[u][j] = q_j exp(-i kappa d_u.(y_j - c)) for the pass's slots u
+      for (int x = tid; x < C * 64; x += 128) {
+        const int u = x >> 6, j = x & 63;
+        R2 val = c2<R>(0, 0);
+        if (j < cn && u < kc) {
+          const R2 qq = q[pb + j0 + j];
+          if (hf) {
+            const int d = S.slotDir[s0 + k0 + u];
+            const double dx = dirs[3 * (doff + d)], dy = dirs[3 * (doff + d) + 1],
+                         dz = dirs[3 * (doff + d) + 2];
+            R cs, sn;
+            expi(-kappa * (dx * rel[0][j] + dy * rel[1][j] + dz * rel[2][j]), cs, sn);
+            val = cmul(qq, c2<R>(cs, sn));
+          } else {
+            val = qq;
+          }
         }
-        R2 qq = q[pb + j0 + j];
-        if (hf) {
-          R cs, sn;
-          expi(-kappa * (dx * rel[0][j] + dy * rel[1][j] + dz * rel[2][j]), cs, sn);
-          w[j] = cmul(qq, c2<R>(cs, sn));
-        } else {
-          w[j] = qq;
-        }
+        w[u][j] = val;
       }
       __syncthreads();
 #pragma unroll
@@ -154,13 +162,16 @@ __global__ __launch_bounds__(128) void k_p2m(const int* __restrict__ leaves, Dev
         const int I = tid + 128 * m;
         if (I < LL) {
           const int i0 = I / (L * L), i1 = (I / L) % L, i2 = I % L;
-          R2 a = acc[m];
           for (int j = 0; j < cn; j++) {
-            R s = Sw[0][i0][j] * Sw[1][i1][j] * Sw[2][i2][j];
-            a.x += s * w[j].x;
-            a.y += s * w[j].y;
+            const R s = Sw[0][i0][j] * Sw[1][i1][j] * Sw[2][i2][j];
+#pragma unroll
+            for (int u = 0; u < C; u++) {
+              if (u < kc) {
+                acc[m][u].x += s * w[u][j].x;
+                acc[m][u].y += s * w[u][j].y;
+              }
+            }
           }
-          acc[m] = a;
         }
       }
     }
@@ -168,9 +179,14 @@ __global__ __launch_bounds__(128) void k_p2m(const int* __restrict__ leaves, Dev
     for (int m = 0; m < NI; m++) {
       const int I = tid + 128 * m;
       if (I < LL) {
-        R2 v = acc[m];
-        if (hf) v = cmulc(v, sig2[(size_t)(srow + d) * LL + I]);
-        M[(size_t)(s0 + k) * LL + I] = v;
+#pragma unroll
+        for (int u = 0; u < C; u++) {
+          if (u < kc) {
+            R2 v = acc[m][u];
+            if (hf) v = cmulc(v, sig2[(size_t)(srow + S.slotDir[s0 + k0 + u]) * LL + I]);
+            M[(size_t)(s0 + k0 + u) * LL + I] = v;
+          }
+        }
       }
     }
   }
