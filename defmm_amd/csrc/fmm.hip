@@ -325,6 +325,127 @@ __global__ __launch_bounds__(128) void k_trans(
   }
 }
 
+// The same translation with one lane per line (used where it is the faster
+// of the two: float, and L >= 6 in double).  One wave per output slot; an
+// entry's input row times its pre-phase is staged in LDS; in each sweep lane
+// l < L^2 owns one whole line along the swept axis, reads its L values once
+// and forms all L outputs (k_trans reads L values per output).  The entry's
+// 1D matrix is the same for every lane, so it comes through scalar loads
+// rather than LDS.
+template <typename R, int L>
+__global__ __launch_bounds__(64) void k_trans_lines(
+    const int* __restrict__ out, const int* __restrict__ beg, const int* __restrict__ in,
+    const int* __restrict__ sgn, const int* __restrict__ sigIdx, const double2* __restrict__ edv,
+    const typename C2T<R>::T* __restrict__ src, typename C2T<R>::T* __restrict__ dst,
+    const typename C2T<R>::T* __restrict__ preTab, const typename C2T<R>::T* __restrict__ postTab,
+    const R* __restrict__ mat0, const R* __restrict__ mat1, int accumulate, int nItems) {
+  typedef typename C2T<R>::T R2;
+  constexpr int LL = L * L * L, NL = L * L;
+  constexpr int NI = (LL + 63) / 64;
+  static_assert(NL <= 64, "one lane per line needs L*L <= 64");
+  __shared__ R2 a[LL], b[LL];
+  const int lane = threadIdx.x;
+  // XCD-contiguous output order, as in k_trans
+  const int blk = itemIndex();
+  if (blk >= nItems) return;
+  const int n8 = nItems & ~7;
+  const int o = blk < n8 ? (blk & 7) * (n8 >> 3) + (blk >> 3) : blk;
+  const int e0 = beg[o], e1 = beg[o + 1];
+  // the line of this lane; lanes >= L^2 sweep line 0 and store nothing
+  const bool own = lane < NL;
+  const int ln = own ? lane : 0;
+  const int hi = ln / L, lo = ln % L;
+  R2 acc[L];  // elements I = i * L^2 + ln
+#pragma unroll
+  for (int i = 0; i < L; i++) acc[i] = c2<R>(0, 0);
+  // the input row (and its phase row) of entry e + 1 is loaded into
+  // registers while entry e is swept, hiding the load latency
+  R2 nv[NI], np[NI];
+  auto fetch = [&](int e) {
+    const int s = in[e], row = sigIdx[e];
+#pragma unroll
+    for (int m = 0; m < NI; m++) {
+      const int I = min(lane + 64 * m, LL - 1);
+      nv[m] = src[(size_t)s * LL + I];
+      np[m] = row >= 0 ? preTab[(size_t)row * LL + I] : c2<R>(1, 0);
+    }
+  };
+  // w[i] = sum_k M[i * L + k] * v[k] along one line
+  auto line = [&](const R* __restrict__ M, const R2* v, R2* w) {
+#pragma unroll
+    for (int i = 0; i < L; i++) {
+      R2 t = c2<R>(0, 0);
+#pragma unroll
+      for (int k = 0; k < L; k++) {
+        const R m = M[i * L + k];
+        t.x += m * v[k].x;
+        t.y += m * v[k].y;
+      }
+      w[i] = t;
+    }
+  };
+  if (e0 < e1) fetch(e0);
+  for (int e = e0; e < e1; e++) {
+    const int row = sigIdx[e], sg = sgn[e];
+    const double2 ed = edv[e];
+    const R2 edr = c2<R>((R)ed.x, (R)ed.y);
+    // post-phase of this lane's outputs, loaded ahead of the sweeps
+    R2 pv[L];
+#pragma unroll
+    for (int i = 0; i < L; i++)
+      pv[i] = row >= 0 ? postTab[(size_t)row * LL + i * NL + ln] : c2<R>(1, 0);
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < NI; m++) {
+      const int I = lane + 64 * m;
+      if (I < LL) a[I] = cmul(nv[m], np[m]);
+    }
+    if (e + 1 < e1) fetch(e + 1);
+    __syncthreads();
+    R2 v[L], w[L];
+    // axis 2: line (i0, i1) = (hi, lo), elements ln * L + k
+#pragma unroll
+    for (int k = 0; k < L; k++) v[k] = a[ln * L + k];
+    line((sg & 4) ? mat1 : mat0, v, w);
+    if (own)
+#pragma unroll
+      for (int i = 0; i < L; i++) b[ln * L + i] = w[i];
+    __syncthreads();
+    // axis 1: line (i0, i2) = (hi, lo), elements hi * L^2 + k * L + lo
+#pragma unroll
+    for (int k = 0; k < L; k++) v[k] = b[hi * NL + k * L + lo];
+    line((sg & 2) ? mat1 : mat0, v, w);
+    if (own)
+#pragma unroll
+      for (int i = 0; i < L; i++) a[hi * NL + i * L + lo] = w[i];
+    __syncthreads();
+    // axis 0: line (i1, i2) = ln, elements k * L^2 + ln
+#pragma unroll
+    for (int k = 0; k < L; k++) v[k] = a[k * NL + ln];
+    line((sg & 1) ? mat1 : mat0, v, w);
+#pragma unroll
+    for (int i = 0; i < L; i++) {
+      const R2 t = cmul(cmulc(w[i], pv[i]), edr);
+      acc[i].x += t.x;
+      acc[i].y += t.y;
+    }
+  }
+  if (!own) return;
+  const int os = out[o];
+#pragma unroll
+  for (int i = 0; i < L; i++) {
+    const int I = i * NL + ln;
+    R2 v = acc[i];
+    if (accumulate) {
+      R2 old = dst[(size_t)os * LL + I];
+      v.x += old.x;
+      v.y += old.y;
+    }
+    dst[(size_t)os * LL + I] = v;
+  }
+}
+
+
 // M2F (M2F.hpp:35-46, circulant.hpp:95-106): F = DFT_P^3 of M embedded in
 // the (2L-1)^3 grid at I2i(I, P), unnormalised, exp(-2 pi i jk/P).  The
 // embedding is zero outside the L^3 corner, so the three axis passes sum over
@@ -1169,6 +1290,21 @@ struct DevTrans {
   double2* edv = nullptr;
 };
 
+// k_trans_lines where it is the faster one (measured: float at every order,
+// double from L = 6; k_trans for double at L <= 5)
+template <typename R, int L>
+static void launchTrans(const DevTrans& t, const typename C2T<R>::T* src,
+                        typename C2T<R>::T* dst, const typename C2T<R>::T* pre,
+                        const typename C2T<R>::T* post, const R* m0, const R* m1, int accumulate,
+                        hipStream_t s) {
+  if (sizeof(R) == 4 || L >= 6)
+    hipLaunchKernelGGL((k_trans_lines<R, L>), itemGrid(t.n), dim3(64), 0, s, t.out, t.beg, t.in,
+                       t.sgn, t.sigIdx, t.edv, src, dst, pre, post, m0, m1, accumulate, t.n);
+  else
+    hipLaunchKernelGGL((k_trans<R, L>), itemGrid(t.n), dim3(128), 0, s, t.out, t.beg, t.in,
+                       t.sgn, t.sigIdx, t.edv, src, dst, pre, post, m0, m1, accumulate, t.n);
+}
+
 struct Exec {
   int prec = 0;
   int L = 0, P = 0, LL = 0, PP = 0;
@@ -1784,9 +1920,7 @@ static int runTyped(Exec& X, const double2* qin, double2* pout, int parts, int n
   if (far)
     for (int r = 0; r < nrhs; r++)
       for (auto& t : X.m2m)
-        hipLaunchKernelGGL((k_trans<R, L>), itemGrid(t.n), dim3(128), 0, s, t.out, t.beg, t.in,
-                           t.sgn, t.sigIdx, t.edv, (const R2*)Mr(r), Mr(r), sig, sig2, oc.A[0],
-                           oc.A[1], 0, t.n);
+        launchTrans<R, L>(t, (const R2*)Mr(r), Mr(r), sig, sig2, oc.A[0], oc.A[1], 0, s);
   mark(3);
   const bool stored = X.m2lMode == DEFMM_M2L_STORED;
   if (far && X.nm2f && stored)
@@ -1827,9 +1961,7 @@ static int runTyped(Exec& X, const double2* qin, double2* pout, int parts, int n
   if (far)
     for (int r = 0; r < nrhs; r++)
       for (auto& t : X.l2l)
-        hipLaunchKernelGGL((k_trans<R, L>), itemGrid(t.n), dim3(128), 0, s, t.out, t.beg, t.in,
-                           t.sgn, t.sigIdx, t.edv, (const R2*)Lr(r), Lr(r), sig2, sig, oc.B[0],
-                           oc.B[1], 1, t.n);
+        launchTrans<R, L>(t, (const R2*)Lr(r), Lr(r), sig2, sig, oc.B[0], oc.B[1], 1, s);
   mark(6);
   if (far && X.nl2p)
     for (int r = 0; r < nrhs; r++)
