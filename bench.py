@@ -90,26 +90,31 @@ def gate_regime(name, steps, warmup, samples, nrhs):
         ms = e0.elapsed_time(e1) / steps
         p = Pd[0].cpu().numpy()[idx]
         err = float(np.sqrt((np.abs(p - exact) ** 2).sum() / (np.abs(exact) ** 2).sum()))
-        return ms, err
+        # per-stage times of the last step, and the M2L variant that ran
+        st = {k: round(v, 3) for k, v in A.stage_times().items()}
+        st["m2l_variant"] = ["on the fly", "stored", "batched"][A.m2l_mode()]
+        return ms, err, st
 
-    ms64, e64 = timed(1)
+    ms64, e64, st64 = timed(1)
     A.set_precision(dm.MIXED)
-    msmx, emx = timed(1)
+    msmx, emx, stmx = timed(1)
     A.set_rhs(nrhs)
-    msb, eb = timed(nrhs)
+    msb, eb, stb = timed(nrhs)
     A.destroy()
     out = {"workload": f"{name}: {n:,} points ({geo}), {nl:g} wavelengths, order {L}, "
                        f"leaf<={ncrit - 1}",
            "setup_s": setup_s, "targets": samples,
-           "fp64": {"value": n / (ms64 * 1e-3) / 1e6, "ms_per_step": ms64, "rel_l2_err": e64},
+           "fp64": {"value": n / (ms64 * 1e-3) / 1e6, "ms_per_step": ms64, "rel_l2_err": e64,
+                    "stage_ms": st64},
            "mixed": {"value": n / (msmx * 1e-3) / 1e6, "ms_per_step": msmx, "rel_l2_err": emx,
-                     "speedup_vs_fp64": ms64 / msmx},
+                     "speedup_vs_fp64": ms64 / msmx, "stage_ms": stmx},
            "mixed_batched": {"value": n * nrhs / (msb * 1e-3) / 1e6, "nrhs": nrhs,
                              "ms_per_step": msb, "rel_l2_err_first_rhs": eb,
-                             "speedup_vs_fp64": ms64 * nrhs / msb},
+                             "speedup_vs_fp64": ms64 * nrhs / msb, "stage_ms": stb},
            "unit": "Mpts·RHS/s"}
     log(f"[gate {name}] fp64 {ms64:.2f} ms (err {e64:.3e}), mixed {msmx:.2f} ms (err {emx:.3e}), "
-        f"mixed x{nrhs} {msb:.2f} ms (err {eb:.3e})")
+        f"mixed x{nrhs} {msb:.2f} ms (err {eb:.3e}); stages fp64 {st64}, mixed {stmx}, "
+        f"mixed x{nrhs} {stb}")
     return out
 
 
