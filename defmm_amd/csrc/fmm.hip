@@ -450,10 +450,19 @@ __global__ __launch_bounds__(64) void k_trans_lines(
 // the (2L-1)^3 grid at I2i(I, P), unnormalised, exp(-2 pi i jk/P).  The
 // embedding is zero outside the L^3 corner, so the three axis passes sum over
 // L inputs only.
-// kM2FRows rows per block: their M rows are loaded together at the start (the
-// single-row kernel spent most of its time waiting for that load), then the
-// rows are transformed one after another in LDS.
-constexpr int kM2FRows = 4;
+// Rows per block: as many as fit the two intermediate arrays in 48 KB of LDS
+// (at most 4).
+template <typename R, int L>
+constexpr int m2fRows() {
+  constexpr int P = 2 * L - 1;
+  constexpr int bytes = (L * L * P + L * P * P) * 2 * (int)sizeof(R);
+  constexpr int r = 49152 / bytes;
+  return r < 1 ? 1 : r > 4 ? 4 : r;
+}
+// Each pass gives one lane a whole line along its axis: the lane reads the
+// line's L inputs once and forms all P outputs, with twiddles whose indices
+// are compile-time constants (held in registers, not read from LDS); the last
+// pass stores F straight from registers.
 template <typename R, int L>
 __global__ __launch_bounds__(256) void k_m2f(const int* __restrict__ m2fSlot,
                                              const typename C2T<R>::T* __restrict__ M,
@@ -461,61 +470,68 @@ __global__ __launch_bounds__(256) void k_m2f(const int* __restrict__ m2fSlot,
                                              const R* __restrict__ twc,
                                              const R* __restrict__ tws, int nRows) {
   typedef typename C2T<R>::T R2;
-  constexpr int P = 2 * L - 1, LL = L * L * L, PP = P * P * P;
-  __shared__ R2 sM[kM2FRows][LL], T1[L * L * P], T2[L * P * P];
-  __shared__ R wc[P], ws[P];
+  constexpr int P = 2 * L - 1, LL = L * L * L, PP = P * P * P, NR = m2fRows<R, L>();
+  __shared__ R2 T1[NR][L * L * P], T2[NR][L * P * P];
   const int tid = threadIdx.x;
-  const int row0 = itemIndex() * kM2FRows;
+  const int row0 = itemIndex() * NR;
   if (row0 >= nRows) return;
-  const int nr = min(kM2FRows, nRows - row0);
-  if (tid < P) {
-    wc[tid] = twc[tid];
-    ws[tid] = tws[tid];
+  const int nr = min(NR, nRows - row0);
+  R cw[P], sw[P];
+#pragma unroll
+  for (int m = 0; m < P; m++) {
+    cw[m] = twc[m];
+    sw[m] = tws[m];
   }
-  for (int x = tid; x < nr * LL; x += 256) {
-    const int k = x / LL, I = x - k * LL;
-    sM[k][I] = M[(size_t)m2fSlot[row0 + k] * LL + I];
+  // t += v * exp(-2 pi i m / P)
+  auto mac = [&](R2& t, const R2 v, const int m) {
+    t.x += v.x * cw[m] + v.y * sw[m];
+    t.y += v.y * cw[m] - v.x * sw[m];
+  };
+  // axis 2: line (k, i0, i1) of M -> T1[k][(i0 * L + i1) * P + k2]
+  for (int x = tid; x < nr * L * L; x += 256) {
+    const int k = x / (L * L), r = x - k * (L * L);
+    const R2* Mr = M + (size_t)m2fSlot[row0 + k] * LL + r * L;
+    R2 v[L];
+#pragma unroll
+    for (int j = 0; j < L; j++) v[j] = Mr[j];
+#pragma unroll
+    for (int k2 = 0; k2 < P; k2++) {
+      R2 t = c2<R>(0, 0);
+#pragma unroll
+      for (int j = 0; j < L; j++) mac(t, v[j], (j * k2) % P);
+      T1[k][r * P + k2] = t;
+    }
   }
-  for (int k = 0; k < nr; k++) {
-    __syncthreads();
-    for (int x = tid; x < L * L * P; x += 256) {
-      const int k2 = x % P, r = x / P;
+  __syncthreads();
+  // axis 1: line (k, i0, k2) -> T2[k][(i0 * P + k1) * P + k2]
+  for (int x = tid; x < nr * L * P; x += 256) {
+    const int k = x / (L * P), r = x - k * (L * P);
+    const int i0 = r / P, k2 = r - i0 * P;
+    R2 v[L];
+#pragma unroll
+    for (int j = 0; j < L; j++) v[j] = T1[k][(i0 * L + j) * P + k2];
+#pragma unroll
+    for (int k1 = 0; k1 < P; k1++) {
       R2 t = c2<R>(0, 0);
 #pragma unroll
-      for (int j = 0; j < L; j++) {
-        const int m = (j * k2) % P;
-        const R2 v = sM[k][r * L + j];
-        t.x += v.x * wc[m] + v.y * ws[m];
-        t.y += v.y * wc[m] - v.x * ws[m];
-      }
-      T1[x] = t;
+      for (int j = 0; j < L; j++) mac(t, v[j], (j * k1) % P);
+      T2[k][(i0 * P + k1) * P + k2] = t;
     }
-    __syncthreads();
-    for (int x = tid; x < L * P * P; x += 256) {
-      const int k2 = x % P, k1 = (x / P) % P, i0 = x / (P * P);
+  }
+  __syncthreads();
+  // axis 0: line (k, k1, k2) -> F[row0 + k][k0 * P^2 + k1 * P + k2]
+  for (int x = tid; x < nr * P * P; x += 256) {
+    const int k = x / (P * P), r = x - k * (P * P);
+    R2 v[L];
+#pragma unroll
+    for (int j = 0; j < L; j++) v[j] = T2[k][j * P * P + r];
+    R2* Fr = F + (size_t)(row0 + k) * PP + r;
+#pragma unroll
+    for (int k0 = 0; k0 < P; k0++) {
       R2 t = c2<R>(0, 0);
 #pragma unroll
-      for (int j = 0; j < L; j++) {
-        const int m = (j * k1) % P;
-        const R2 v = T1[(i0 * L + j) * P + k2];
-        t.x += v.x * wc[m] + v.y * ws[m];
-        t.y += v.y * wc[m] - v.x * ws[m];
-      }
-      T2[x] = t;
-    }
-    __syncthreads();
-    R2* Fr = F + (size_t)(row0 + k) * PP;
-    for (int x = tid; x < PP; x += 256) {
-      const int k0 = x / (P * P), r = x % (P * P);
-      R2 t = c2<R>(0, 0);
-#pragma unroll
-      for (int j = 0; j < L; j++) {
-        const int m = (j * k0) % P;
-        const R2 v = T2[j * P * P + r];
-        t.x += v.x * wc[m] + v.y * ws[m];
-        t.y += v.y * wc[m] - v.x * ws[m];
-      }
-      Fr[x] = t;
+      for (int j = 0; j < L; j++) mac(t, v[j], (j * k0) % P);
+      Fr[k0 * P * P] = t;
     }
   }
 }
@@ -1925,7 +1941,7 @@ static int runTyped(Exec& X, const double2* qin, double2* pout, int parts, int n
   const bool stored = X.m2lMode == DEFMM_M2L_STORED;
   if (far && X.nm2f && stored)
     for (int r = 0; r < nrhs; r++)
-      hipLaunchKernelGGL((k_m2f<R, L>), itemGrid((X.nm2f + kM2FRows - 1) / kM2FRows), dim3(256), 0, s,
+      hipLaunchKernelGGL((k_m2f<R, L>), itemGrid((X.nm2f + m2fRows<R, L>() - 1) / m2fRows<R, L>()), dim3(256), 0, s,
                          X.m2fSlot,
                          (const R2*)Mr(r), Fr(r), oc.twc, oc.tws, X.nm2f);
   mark(4);
@@ -1946,7 +1962,7 @@ static int runTyped(Exec& X, const double2* qin, double2* pout, int parts, int n
         const int g0 = X.hBatchGroupBeg[b], g1 = X.hBatchGroupBeg[b + 1];
         const int w0 = X.hBatchRowBeg[b], w1 = X.hBatchRowBeg[b + 1];
         for (int r = 0; r < nrhs; r++)
-          hipLaunchKernelGGL((k_m2f<R, L>), itemGrid((w1 - w0 + kM2FRows - 1) / kM2FRows), dim3(256),
+          hipLaunchKernelGGL((k_m2f<R, L>), itemGrid((w1 - w0 + m2fRows<R, L>() - 1) / m2fRows<R, L>()), dim3(256),
                              0, s, X.bRows + w0,
                              (const R2*)Mr(r), Fr(r), oc.twc, oc.tws, w1 - w0);
         for (int r0 = 0; r0 < nrhs;) r0 += m2lChunk<R, L>(X, s, g0, g1 - g0, r0, nrhs - r0, oc);
