@@ -580,12 +580,22 @@ struct M2LShapeN {
 };
 // Right-hand sides per M2L launch: at most 16 (double) or 32 (float)
 // complex accumulators per thread.
-// (double at L = 8: one, the 2-wide kernel spills)
+// (double from L = 7: one, float at L = 7: two; wider kernels spill at the
+// occupancy asked for below, and at L = 8 in double without it)
 template <typename R, int L>
 constexpr int m2lMaxNR() {
   constexpr int NW = M2LShape<L>::NW;
   constexpr int cap = (sizeof(R) == 8 ? 16 : 32) / (M2LShape<L>::G * NW);
-  return (sizeof(R) == 8 && L >= 8) ? 1 : cap >= 8 ? 8 : cap >= 4 ? 4 : cap >= 2 ? 2 : 1;
+  constexpr int w = cap >= 8 ? 8 : cap >= 4 ? 4 : cap >= 2 ? 2 : 1;
+  return (sizeof(R) == 8 && L >= 7) ? 1 : (L == 7 && w > 2) ? 2 : w;
+}
+// Waves per SIMD the compiler is asked to fit: at L = 7 (768-thread blocks)
+// float with two right-hand sides took 105 VGPRs, one block per CU; 6 waves
+// per SIMD (two blocks) fit without spilling in groups of 3 targets (in
+// double they spill, so double keeps one block per CU).
+template <typename R, int L, int NR>
+constexpr int m2lWavesPerEU() {
+  return (L == 7 && sizeof(R) == 4) ? 6 : 1;
 }
 
 template <int PP>
@@ -597,7 +607,8 @@ __device__ __forceinline__ size_t tableRow(int code) {
 // NR right-hand sides per launch: right-hand side r reads F + r * fStride
 // and writes Lout + r * lStride; each table value is loaded once for all.
 template <typename R, int L, int NR>
-__global__ __launch_bounds__((M2LShapeN<L, NR>::TB)) void k_m2l(
+__global__ __launch_bounds__((M2LShapeN<L, NR>::TB))
+__attribute__((amdgpu_waves_per_eu(m2lWavesPerEU<R, L, NR>(), 8))) void k_m2l(
     const int* __restrict__ groupBeg, const int* __restrict__ groupTarget,
     const int* __restrict__ gSrcF, const int* __restrict__ gMask, const int* __restrict__ gCode,
     const typename C2T<R>::T* __restrict__ F, size_t fStride,

@@ -18,9 +18,12 @@ typedef std::complex<double> cplx;
 
 // Target slots per M2L work group: a thread holds one accumulator per group
 // target and frequency it owns (2 targets at L = 8, where a thread owns 4
-// frequencies, so that the FP64 kernel fits its registers).
+// frequencies, so that the FP64 kernel fits its registers; 3 at L = 7, where
+// the float kernel with two right-hand sides then fits two 768-thread blocks
+// per CU: 1M points / 16 wavelengths, 4 right-hand sides 182.8 -> 159.8 ms,
+// FP64 89.1 -> 91.4 ms).
 constexpr int kM2LGroupMax = 8;
-constexpr int m2lGroupSize(int L) { return L >= 8 ? 2 : 4; }
+constexpr int m2lGroupSize(int L) { return L >= 8 ? 2 : L == 7 ? 3 : 4; }
 
 // Error codes returned through the C ABI.
 enum {
