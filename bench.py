@@ -155,7 +155,7 @@ def main():
                          "traffic, to scale rocprof byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path "
+        print(f"bench.py measures one GPU: the library has no multi-GPU path "
               f"(requested --gpus {args.gpus}); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 

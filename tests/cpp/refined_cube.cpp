@@ -1,4 +1,4 @@
-// The reference's driver scenario (test/testinterface.cpp of the reference)
+// The reference's test program (test/testinterface.cpp of the reference)
 // written against the drop-in header: refined-cube mesh as sources and
 // targets, rand()/RAND_MAX real charges, prcmpt(ORDER, NCRIT, 1), gemv,
 // CheckFULL.  argv: mesh path, ORDER, NCRIT.
