@@ -1,11 +1,13 @@
 """GPU parity on reduced-size versions of the benchmark configurations the
 headline run does not cover: the clustered 4:1:1 ellipsoid in FP64 and mixed
 precision (L = 5, 6, 7), several right-hand sides on the ellipsoid (L = 6,
-leaf <= 96), and a batch of uneven instances (geometry, size, wavelengths,
-order and right-hand sides drawn per instance).  Every result is compared
-with the oracle on the same inputs: FP64 to relative L2 1e-12 / max 1e-10,
-mixed to relative L2 3e-6.  Errors against the FP64 direct sum on sampled
-targets are printed.
+leaf <= 96; L = 7), and a batch of uneven instances (geometry, size,
+wavelengths, order and right-hand sides drawn per instance).  Results are
+compared with the oracle on the same inputs (FP64 to relative L2 1e-12 / max
+1e-10, mixed to relative L2 3e-6), except the L = 7 right-hand-side test,
+which compares batches with single evaluations (the L = 7 single evaluation
+is the one pinned to the oracle above).  Errors against the FP64 direct sum on
+sampled targets are printed.
 """
 import sys
 
@@ -81,6 +83,33 @@ def test_ellipsoid_multi_rhs_vs_oracle():
             l2, mx = _diff(P[r], ref[r])
             _say(f"  precision {prec} rhs {r}: vs oracle rel L2 {l2:.3e} max {mx:.3e}")
             assert l2 <= tol
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_ellipsoid_multi_rhs_L7_matches_single(mode):
+    """Order 7 with several right-hand sides: the float M2L kernels take 2
+    right-hand sides per launch there (groups of 3 targets, compiled for 6
+    waves per SIMD), FP64 one; 3 right-hand sides run as 2 + 1 and 1 + 1 + 1.
+    Each must equal its single-right-hand-side evaluation, with the stored
+    (mode 1) and the batched (mode 2) M2L variant."""
+    import defmm_amd as dm
+    from defmm_amd import workloads as wl
+    n, nr = 8000, 3
+    pts = wl.ellipsoid_clusters(n, seed=3)
+    kappa = wl.kappa_for(16, pts)
+    Q = np.stack([wl.charges(n, seed=80 + r) for r in range(nr)])
+    for prec, tol in ((dm.FP64, 1e-13), (dm.MIXED, 1e-6)):
+        _say(f"case: clustered ellipsoid N={n} 16 wavelengths L=7, {nr} right-hand sides, "
+             f"M2L mode {mode}, precision {prec}")
+        A = _plan(dm, pts, 7, 65, kappa, prec)
+        A.set_m2l(mode)
+        single = np.stack([A.apply(Q[r]) for r in range(nr)])
+        A.set_rhs(nr)
+        P = A.apply_batch(Q)
+        A.destroy()
+        l2, mx = _diff(P, single)
+        _say(f"  batch vs single rel L2 {l2:.3e} max {mx:.3e}")
+        assert l2 <= tol and mx <= 10 * tol
 
 
 def test_uneven_instance_batch():
