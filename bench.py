@@ -1,18 +1,42 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X FMM evaluation pass (one JSON line on stdout).
 
-Workload (default): 1,000,000 points uniform on the unit sphere, 128
-wavelengths across the box diameter, interpolation order L = 5, leaves of at
-most 64 points (Ncrit = 65), one complex right-hand side with re, im ~ U(-1, 1),
-FP64 -- the reference's gemv (include/interface.hpp:153-170) as a drop-in.
-A step = one evaluation pass with charges and potentials resident in HBM;
-setup (octree, lists, tables, upload) is timed separately.  The same plan is
-then switched to the mixed-precision path and timed again.
+Metric: Helmholtz FMM throughput in Mpts*RHS/s (targets x right-hand sides
+per second of evaluation) at relative error <= 1e-6 against the FP64 direct
+sum, mixed-precision batched path against the plain FP64 path.
 
-Accuracy: relative L2 / L-inf error on 2,000 sampled targets against an FP64
-direct sum computed on the GPU.  CPU baseline: the oracle (a serial FP64
-restatement of the reference, one thread) on a bounded sample of the same
-family: 20,000 sphere points at the same points-per-wavelength density.
+Headline workload (default): 20,000 points on the surface of the unit cube,
+refined toward the edges, 32 wavelengths across the box, leaves of at most 64
+points, complex charges re, im ~ U(-1, 1), at interpolation order 8 -- the
+lowest order whose error is <= 1e-6 on this input (7.2e-7; order 7 gives
+5.8e-6, order 4 of the workload's nominal setting 1.1e-3).  A step is one
+evaluation pass (the reference's gemv, include/interface.hpp:153-170) with
+charges and potentials resident in HBM; setup (octree, lists, tables,
+upload) is timed separately.  Measured on one plan:
+  fp64           FP64, one right-hand side (the drop-in)
+  mixed          float expansions and near field, double phase reduction
+  mixed_batched  the same with --nrhs right-hand sides per pass: `value`
+Errors: relative L2 / L-inf on --samples sampled targets against an FP64
+direct sum computed on the GPU.
+
+Scale section (--scale-workload, default the 1M-point sphere at 128
+wavelengths, order 5): FP64 and mixed at full size; at this frequency order 5
+gives 7e-3, and the order that reaches 1e-6 does not fit in one GPU's memory
+(see the design notes), so it reports throughput and error as measured.
+
+Roofline: the dominant kernel of the value pass.  Compulsory HBM bytes of
+the fused M2L+F2L kernel = Fourier rows read once per right-hand side + kernel
+table rows read once per launch + local expansions written once per
+right-hand side + the group lists; divided by the stage's time measured with
+HIP events on the launch stream.  `traffic` is the measured FETCH_SIZE +
+WRITE_SIZE per launch from a rocprofv3 counter run (profiles/), scaled by a
+copy kernel of known size.
+
+CPU baseline: the oracle (serial FP64 restatement of the reference) at order
+8 on a reduced twin of the workload (8,000 cube points at the same points per
+wavelength squared), timed with one thread and with one evaluation per core
+running concurrently (the reference is serial and holds one instance per
+process).
 """
 import argparse
 import json
@@ -36,86 +60,180 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(n=20000, L=5, ncrit=65):
-    """Oracle (port) on a sphere sample at the workload's surface density."""
+def m2l_nr_per_launch(L, mixed):
+    """Right-hand sides one M2L launch serves (the executor's rule: at most 16
+    double or 32 float complex accumulators per thread; 1 in double from
+    order 7, 2 in float at order 7)."""
+    PP = (2 * L - 1) ** 3
+    nw = max(-(-PP // 1024), min(-(-PP // 256), 3))
+    g = 2 if L >= 8 else 3 if L == 7 else 4
+    cap = (32 if mixed else 16) // (g * nw)
+    w = 8 if cap >= 8 else 4 if cap >= 4 else 2 if cap >= 2 else 1
+    if not mixed and L >= 7:
+        return 1
+    if L == 7 and w > 2:
+        return 2
+    return w
+
+
+def m2l_compulsory_bytes(st, L, nrhs, mixed):
+    """Compulsory HBM bytes of the M2L+F2L stage (stored Fourier rows) for
+    nrhs right-hand sides: every F row read once per right-hand side, every
+    table row once per launch, every local expansion written once per
+    right-hand side, and the group lists once per launch."""
+    PP, LL = (2 * L - 1) ** 3, L ** 3
+    s = 8 if mixed else 16
+    g = 2 if L >= 8 else 3 if L == 7 else 4
+    nr = m2l_nr_per_launch(L, mixed)
+    launches = -(-nrhs // nr)
+    lists = st["m2l_group_entries"] * (4 + 4 + 4 * g) + st["m2l_groups"] * (4 + 4 * g)
+    per_rhs = st["m2f_slots"] * PP * s + st["m2l_targets"] * LL * s
+    per_launch = st["m2l_table_rows"] * PP * s + lists
+    return nrhs * per_rhs + launches * per_launch, launches
+
+
+def cpu_info():
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    return model, os.cpu_count(), avail
+
+
+def cpu_baseline(L=8, n=8000, cores=None):
+    """Oracle (port) at order L on a reduced twin of the headline workload:
+    n cube points at the same points per wavelength squared; one thread, then
+    `cores` concurrent evaluations (different charges), apply time only."""
+    from concurrent.futures import ThreadPoolExecutor
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_api as oa
     from defmm_amd import workloads as wl
-    nl = 128.0 * np.sqrt(n / 1.0e6)
-    pts = wl.sphere(n, seed=1)
+    model, nproc, avail = cpu_info()
+    if cores is None:
+        cores = min(16, avail)
+    nl = 32.0 * np.sqrt(n / 20000.0)
+    pts = wl.cube_edges(n, seed=1)
     kappa = wl.kappa_for(nl, pts)
-    q = wl.charges(n, seed=7)
-    os.environ.setdefault("OMP_NUM_THREADS", "1")
-    _, _, ts, ta = oa.fmm(pts, pts, q, L, ncrit, kappa)
-    return {"value": n / ta / 1e6, "unit": "Mpts·RHS/s", "cores": 1, "kind": "port",
-            "sample": f"sphere N={n} at {nl:.2f} wavelengths (same points per wavelength^2 as "
-                      f"the workload), L={L}, leaf<=64, FP64, evaluation only "
-                      f"({ta:.1f} s; setup {ts:.1f} s excluded)"}
+    _, st, ts, ta1 = oa.fmm(pts, pts, wl.charges(n, seed=7), L, 65, kappa)
+    one = n / ta1 / 1e6
 
+    def job(r):
+        return oa.fmm(pts, pts, wl.charges(n, seed=900 + r), L, 65, kappa)[3]
 
-def gate_regime(name, steps, warmup, samples, nrhs):
-    """The same measurement, compact, on a workload where this method reaches
-    relative error <= 1e-6 (order 7 at 16 wavelengths): FP64, mixed, mixed
-    with nrhs right-hand sides; errors against the GPU FP64 direct sum."""
-    import torch
-    import defmm_amd as dm
-    from defmm_amd import workloads as wl
-    pts, kappa, L, ncrit = wl.make(name, seed=1)
-    geo, _, nl, _, _ = wl.CONFIGS[name]
-    n = len(pts)
-    Q = np.stack([wl.charges(n, seed=7)] + [wl.charges(n, seed=100 + r) for r in range(1, nrhs)])
-    A = dm.IBFMM_Mat(precision=dm.FP64)
-    A.setSourceParticles(pts)
-    A.setTargetParticles(pts)
     t0 = time.time()
-    A.prcmpt(L, ncrit, kappa)
-    setup_s = time.time() - t0
-    idx = np.linspace(0, n - 1, samples).astype(np.int64)
-    exact = dm.direct(pts, Q[0], pts[idx], kappa)
-    dev = torch.device("cuda:0")
-    stream = torch.cuda.current_stream(dev)
-    Qd = torch.from_numpy(Q).to(dev)
-    Pd = torch.zeros((nrhs, n), dtype=torch.complex128, device=dev)
+    with ThreadPoolExecutor(cores) as ex:
+        tas = list(ex.map(job, range(cores)))
+    wall = time.time() - t0
+    agg = sum(n / t / 1e6 for t in tas)
+    return {"value": agg, "unit": "Mpts·RHS/s", "cores": cores, "kind": "port",
+            "value_1_thread": one, "cpu_model": model, "nproc": nproc,
+            "cores_available": avail,
+            "sample": f"oracle at order {L} on {n:,} edge-refined cube points at {nl:.2f} "
+                      f"wavelengths (the workload's points per wavelength^2), leaf<=64, FP64, "
+                      f"evaluation only: 1 thread {ta1:.1f} s; {cores} concurrent evaluations "
+                      f"(one per core, different charges) {np.mean(tas):.1f} s each, "
+                      f"{wall:.1f} s wall; setup {ts:.1f} s excluded"}
 
-    def timed(nr):
-        for _ in range(warmup):
-            A.apply_batch_device(Qd.data_ptr(), Pd.data_ptr(), nr, stream.cuda_stream)
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        for _ in range(steps):
-            A.apply_batch_device(Qd.data_ptr(), Pd.data_ptr(), nr, stream.cuda_stream)
-        e1.record(stream)
-        torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / steps
-        p = Pd[0].cpu().numpy()[idx]
-        err = float(np.sqrt((np.abs(p - exact) ** 2).sum() / (np.abs(exact) ** 2).sum()))
-        # per-stage times of the last step, and the M2L variant that ran
-        st = {k: round(v, 3) for k, v in A.stage_times().items()}
-        st["m2l_variant"] = ["on the fly", "stored", "batched"][A.m2l_mode()]
-        return ms, err, st
 
-    ms64, e64, st64 = timed(1)
-    A.set_precision(dm.MIXED)
-    msmx, emx, stmx = timed(1)
-    A.set_rhs(nrhs)
-    msb, eb, stb = timed(nrhs)
-    A.destroy()
-    out = {"workload": f"{name}: {n:,} points ({geo}), {nl:g} wavelengths, order {L}, "
-                       f"leaf<={ncrit - 1}",
-           "setup_s": setup_s, "targets": samples,
-           "fp64": {"value": n / (ms64 * 1e-3) / 1e6, "ms_per_step": ms64, "rel_l2_err": e64,
-                    "stage_ms": st64},
-           "mixed": {"value": n / (msmx * 1e-3) / 1e6, "ms_per_step": msmx, "rel_l2_err": emx,
-                     "speedup_vs_fp64": ms64 / msmx, "stage_ms": stmx},
-           "mixed_batched": {"value": n * nrhs / (msb * 1e-3) / 1e6, "nrhs": nrhs,
-                             "ms_per_step": msb, "rel_l2_err_first_rhs": eb,
-                             "speedup_vs_fp64": ms64 * nrhs / msb, "stage_ms": stb},
-           "unit": "Mpts·RHS/s"}
-    log(f"[gate {name}] fp64 {ms64:.2f} ms (err {e64:.3e}), mixed {msmx:.2f} ms (err {emx:.3e}), "
-        f"mixed x{nrhs} {msb:.2f} ms (err {eb:.3e}); stages fp64 {st64}, mixed {stmx}, "
-        f"mixed x{nrhs} {stb}")
-    return out
+class Runner:
+    def __init__(self, name, samples, steps, warmup):
+        import torch
+        import defmm_amd as dm
+        from defmm_amd import workloads as wl
+        self.torch, self.dm, self.wl = torch, dm, wl
+        self.name, self.steps, self.warmup = name, steps, warmup
+        self.pts, self.kappa, self.L, self.ncrit = wl.make(name, seed=1)
+        geo, _, self.nl, _, _ = wl.CONFIGS[name]
+        self.geo = geo
+        self.n = len(self.pts)
+        self.q = wl.charges(self.n, seed=7)
+        log(f"workload {name}: N={self.n} kappa={self.kappa:.6f} L={self.L} Ncrit={self.ncrit}")
+        A = dm.IBFMM_Mat(precision=dm.FP64)
+        A.setSourceParticles(self.pts)
+        A.setTargetParticles(self.pts)
+        t0 = time.time()
+        A.prcmpt(self.L, self.ncrit, self.kappa)
+        self.setup_s = time.time() - t0
+        self.A = A
+        self.st = A.stats()
+        self.setup_ms = {k[3:]: self.st[k] for k in self.st if k.startswith("ms_")}
+        log(f"setup {self.setup_s:.1f} s ({self.setup_ms} ms), device "
+            f"{A.device_bytes() / 1e9:.1f} GB, M2L variant {A.m2l_mode()}")
+        self.dev = torch.device("cuda:0")
+        self.stream = torch.cuda.current_stream(self.dev)
+        self.idx = np.linspace(0, self.n - 1, samples).astype(np.int64)
+        self.exact = dm.direct(self.pts, self.q, self.pts[self.idx], self.kappa)
+
+    def describe(self):
+        return (f"{self.name}: {self.n:,} points ({self.geo}), {self.nl:g} wavelengths, "
+                f"order {self.L}, leaf<={self.ncrit - 1}")
+
+    def timed(self, nrhs, tag):
+        """Time `steps` passes of nrhs right-hand sides (device arrays);
+        returns ms per pass, errors of the first right-hand side, mean stage
+        times."""
+        torch, A = self.torch, self.A
+        Q = np.stack([self.q] + [self.wl.charges(self.n, seed=100 + r) for r in range(1, nrhs)])
+        Qd = torch.from_numpy(Q).to(self.dev)
+        Pd = torch.zeros((nrhs, self.n), dtype=torch.complex128, device=self.dev)
+        s = self.stream.cuda_stream
+        for _ in range(self.warmup):
+            A.apply_batch_device(Qd.data_ptr(), Pd.data_ptr(), nrhs, s)
+        torch.cuda.synchronize()
+        stages = []
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(self.stream)
+        for _ in range(self.steps):
+            A.apply_batch_device(Qd.data_ptr(), Pd.data_ptr(), nrhs, s)
+            stages.append(A.stage_times())   # events of this step (synchronises it)
+        e1.record(self.stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / self.steps
+        p = Pd[0].cpu().numpy()[self.idx]
+        ex = self.exact
+        e2 = float(np.sqrt((np.abs(p - ex) ** 2).sum() / (np.abs(ex) ** 2).sum()))
+        einf = float(np.abs(p - ex).max() / np.abs(ex).max())
+        avg = {k: float(np.mean([x[k] for x in stages])) for k in stages[0]}
+        log(f"[{self.name} {tag}] {ms:.3f} ms/pass ({nrhs} rhs), rel L2 {e2:.3e}, Linf "
+            f"{einf:.3e}, stages {({k: round(v, 3) for k, v in avg.items()})}")
+        del Qd, Pd
+        return ms, e2, einf, avg
+
+    def roofline(self, stage_ms, nrhs, mixed):
+        """Roofline of the dominant kernel of a pass (M2L+F2L or P2P)."""
+        st, L = self.st, self.L
+        dom = max(["m2l_f2l", "p2p"], key=lambda k: stage_ms[k])
+        t = stage_ms[dom] * 1e-3
+        if dom == "m2l_f2l" and self.A.m2l_mode() == self.dm.M2L_STORED:
+            comp, launches = m2l_compulsory_bytes(st, L, nrhs, mixed)
+            achieved = comp / t / 1e9
+            pair_bytes = (st["m2l_entries"] + st["m2l_targets"]) * (2 * L - 1) ** 3 * \
+                (8 if mixed else 16) * nrhs
+            return {"kernel": "k_m2l (fused M2L Hadamard + F2L)", "bound": "hbm",
+                    "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                    "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                    "compulsory_bytes_per_launch": comp / launches, "launches": launches,
+                    "avg_ms_per_launch": stage_ms[dom] / launches,
+                    "pair_bytes_effective_GBs": pair_bytes / t / 1e9,
+                    "dtype": "f32" if mixed else "f64"}
+        flops = st["p2p_pairs"] * P2P_FLOPS_PER_PAIR * nrhs
+        peak = PEAK_FP32_TFLOPS if mixed else PEAK_FP64_TFLOPS
+        achieved = flops / t / 1e12
+        return {"kernel": "k_p2p + k_p2p_reduce", "bound": "valu", "achieved": achieved,
+                "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                "algorithmic_flops": flops, "avg_ms": stage_ms[dom]}
+
+    def close(self):
+        self.A.destroy()
 
 
 def read_traffic(kernel, workload):
@@ -131,25 +249,15 @@ def read_traffic(kernel, workload):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="sphere1m_128l")
-    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="cube20k_32l_L8")
+    ap.add_argument("--nrhs", type=int, default=16,
+                    help="right-hand sides of the mixed batched pass (the value)")
+    ap.add_argument("--samples", type=int, default=1000)
+    ap.add_argument("--scale-workload", default="sphere1m_128l",
+                    help="full-size workload timed in FP64 and mixed ('none' to skip)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--samples", type=int, default=2000)
-    ap.add_argument("--m2l", default="auto", choices=["auto", "batched", "stored", "fly"],
-                    help="M2L variant of the timed pass (auto: the library's choice)")
-    ap.add_argument("--batch-rows", type=int, default=None,
-                    help="workspace rows of the batched M2L (library default if unset)")
-    ap.add_argument("--compare-m2l", action="store_true",
-                    help="also time the other M2L variants (FP64)")
-    ap.add_argument("--nrhs", type=int, default=2,
-                    help="right-hand sides of the mixed batched pass (0 to skip); 2 keeps the "
-                         "stored-Fourier M2L at 1M points / 128 wavelengths")
-    ap.add_argument("--gate-nrhs", type=int, default=4,
-                    help="right-hand sides of the batched pass of the 1e-6 workload")
-    ap.add_argument("--gate-workload", default="sphere1m_16l_L7",
-                    help="workload of the relative-error <= 1e-6 measurement ('none' to skip)")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="after timing, copy this many GiB device-to-device (a kernel of known "
                          "traffic, to scale rocprof byte counters)")
@@ -161,142 +269,31 @@ def main():
 
     import torch
     import defmm_amd as dm
-    from defmm_amd import workloads as wl
 
-    pts, kappa, L, ncrit = wl.make(args.workload, seed=1)
-    geo, _, nl, _, _ = wl.CONFIGS[args.workload]
-    n = len(pts)
-    q = wl.charges(n, seed=7)
-    log(f"workload {args.workload}: N={n} kappa={kappa:.6f} L={L} Ncrit={ncrit}")
-
-    A = dm.IBFMM_Mat(precision=dm.FP64)
-    A.setSourceParticles(pts)
-    A.setTargetParticles(pts)
-    t0 = time.time()
-    A.prcmpt(L, ncrit, kappa)
-    setup_s = time.time() - t0
-    modes = {"batched": dm.M2L_BATCHED, "stored": dm.M2L_STORED, "fly": dm.M2L_ON_THE_FLY}
-    if args.m2l == "auto":
-        args.m2l = {v: k for k, v in modes.items()}[A.m2l_mode()]
-    A.set_m2l(modes[args.m2l], batch_rows=args.batch_rows)
-    st = A.stats()
-    setup_ms = {k[3:]: st[k] for k in st if k.startswith("ms_")}
-    log(f"setup {setup_s:.1f} s ({setup_ms} ms), device {A.device_bytes() / 1e9:.1f} GB, stats {st}")
-
-    dev = torch.device("cuda:0")
-    qd = torch.from_numpy(q).to(dev)
-    pd = torch.zeros(n, dtype=torch.complex128, device=dev)
-    stream = torch.cuda.current_stream(dev)
-
-    # accuracy reference: direct sum on sampled targets (GPU, FP64)
-    idx = np.linspace(0, n - 1, args.samples).astype(np.int64)
-    exact = dm.direct(pts, q, pts[idx], kappa)
-
-    def run(tag):
-        for _ in range(args.warmup):
-            A.apply_device(qd.data_ptr(), pd.data_ptr(), stream.cuda_stream)
-        torch.cuda.synchronize()
-        stages = []
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        for _ in range(args.steps):
-            A.apply_device(qd.data_ptr(), pd.data_ptr(), stream.cuda_stream)
-            stages.append(A.stage_times())   # events of this step (synchronises it)
-        e1.record(stream)
-        torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / args.steps
-        p = pd.cpu().numpy()[idx]
-        err2 = float(np.sqrt((np.abs(p - exact) ** 2).sum() / (np.abs(exact) ** 2).sum()))
-        errinf = float(np.abs(p - exact).max() / np.abs(exact).max())
-        avg = {k: float(np.mean([s[k] for s in stages])) for k in stages[0]}
-        log(f"[{tag}] {ms:.2f} ms/step, rel L2 {err2:.3e}, Linf {errinf:.3e}, stages {avg}")
-        return ms, err2, errinf, avg
-
-    ms64, e64, einf64, st64 = run(f"fp64 m2l={args.m2l}")
-    alt = None
-    if args.compare_m2l:
-        alt = {}
-        for name in modes:
-            if name == args.m2l:
-                continue
-            A.set_m2l(modes[name])
-            msa, ea, _, sta = run(f"fp64 m2l={name}")
-            alt[name] = {"ms_per_step": msa, "rel_l2_err": ea, "stage_ms": sta,
-                         "device_gb": A.device_bytes() / 1e9}
-        A.set_m2l(modes[args.m2l])
-    value = n / (ms64 * 1e-3) / 1e6
-
-    # dominant kernel of the FP64 pass and its roofline
-    dom = max(["m2l_f2l", "p2p"], key=lambda k: st64[k])
-    if dom == "m2l_f2l":
-        # Hadamard bytes: (M2L pairs + target slots) x PP complex128, tables
-        # counted once (cache-resident, negligible); the batched variant runs
-        # M2F inside this stage: + F rows x (LL + PP) complex128
-        PP, LL = (2 * L - 1) ** 3, L ** 3
-        alg = (st["m2l_entries"] + st["m2l_targets"]) * PP * 16
-        kernel = "k_m2l (M2L Hadamard + F2L)"
-        if args.m2l == "batched":
-            alg += st["m2l_batch_rows"] * (LL + PP) * 16
-            kernel = (f"k_m2f + k_m2l per batch ({st['m2l_batches']} batches of <= "
-                      f"{args.batch_rows or 16384} F rows): M2F, M2L Hadamard, F2L")
-        achieved = alg / (st64[dom] * 1e-3) / 1e9
-        roof = {"kernel": kernel, "bound": "hbm", "achieved": achieved,
-                "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
-                "traffic": read_traffic(f"m2l_{args.m2l}", args.workload),
-                "algorithmic_bytes": alg, "avg_ms": st64[dom]}
-    else:
-        flops = st["p2p_pairs"] * P2P_FLOPS_PER_PAIR
-        achieved = flops / (st64[dom] * 1e-3) / 1e12
-        roof = {"kernel": "k_p2p (+k_p2p_reduce)", "bound": "valu", "achieved": achieved,
-                "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
-                "traffic": None, "algorithmic_flops": flops, "avg_ms": st64[dom]}
-
-    mixed = None
-    if not args.no_mixed:
-        t0 = time.time()
-        A.set_precision(dm.MIXED)
-        log(f"switch to mixed: {time.time() - t0:.1f} s, device {A.device_bytes() / 1e9:.1f} GB")
-        msmx, emx, einfmx, stmx = run("mixed")
-        mixed = {"value": n / (msmx * 1e-3) / 1e6, "ms_per_step": msmx, "rel_l2_err": emx,
-                 "rel_linf_err": einfmx, "speedup_vs_fp64": ms64 / msmx, "stage_ms": stmx,
-                 "dtype": "f32 storage, f64 phase reduction"}
-    batched = None
-    if not args.no_mixed and args.nrhs > 1:
-        nr = args.nrhs
-        t0 = time.time()
-        A.set_rhs(nr)
-        log(f"width {nr} right-hand sides: {time.time() - t0:.1f} s, device "
-            f"{A.device_bytes() / 1e9:.1f} GB, M2L variant {A.m2l_mode()}")
-        Q = np.stack([q] + [wl.charges(n, seed=100 + r) for r in range(1, nr)])
-        Qd = torch.from_numpy(Q).to(dev)
-        Pd = torch.zeros((nr, n), dtype=torch.complex128, device=dev)
-        for _ in range(args.warmup):
-            A.apply_batch_device(Qd.data_ptr(), Pd.data_ptr(), nr, stream.cuda_stream)
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        stages = []
-        e0.record(stream)
-        for _ in range(args.steps):
-            A.apply_batch_device(Qd.data_ptr(), Pd.data_ptr(), nr, stream.cuda_stream)
-            stages.append(A.stage_times())
-        e1.record(stream)
-        torch.cuda.synchronize()
-        msb = e0.elapsed_time(e1) / args.steps
-        p0 = Pd[0].cpu().numpy()[idx]
-        eb = float(np.sqrt((np.abs(p0 - exact) ** 2).sum() / (np.abs(exact) ** 2).sum()))
-        avg = {k: float(np.mean([s[k] for s in stages])) for k in stages[0]}
-        log(f"[mixed x{nr}] {msb:.2f} ms/pass, {msb / nr:.2f} ms per right-hand side, rel L2 "
-            f"(first) {eb:.3e}, stages {avg}")
-        batched = {"value": n * nr / (msb * 1e-3) / 1e6, "nrhs": nr, "ms_per_step": msb,
-                   "rel_l2_err_first_rhs": eb, "speedup_vs_fp64": ms64 * nr / msb,
-                   "stage_ms": avg, "m2l_variant": ["on the fly", "stored", "batched"][A.m2l_mode()],
-                   "device_gb": A.device_bytes() / 1e9,
-                   "dtype": "f32 storage, f64 phase reduction"}
-        del Qd, Pd
-    A.destroy()
+    R = Runner(args.workload, args.samples, args.steps, args.warmup)
+    ms64, e64, einf64, st64 = R.timed(1, "fp64")
+    roof64 = R.roofline(st64, 1, False)
+    R.A.set_precision(dm.MIXED)
+    msmx, emx, einfmx, stmx = R.timed(1, "mixed")
+    R.A.set_rhs(args.nrhs)
+    msb, eb, einfb, stb = R.timed(args.nrhs, f"mixed x{args.nrhs}")
+    roof = R.roofline(stb, args.nrhs, True)
+    roof["traffic"] = read_traffic("m2l_mixed", args.workload)
+    if roof["traffic"]:
+        roof["traffic_over_compulsory"] = roof["traffic"] / roof["compulsory_bytes_per_launch"]
+    variant = ["on the fly", "stored", "batched"][R.A.m2l_mode()]
+    n = R.n
+    v64 = n / (ms64 * 1e-3) / 1e6
+    vmx = n / (msmx * 1e-3) / 1e6
+    value = n * args.nrhs / (msb * 1e-3) / 1e6
+    head = {"workload": R.describe(), "setup_s": R.setup_s, "setup_ms_by_phase": R.setup_ms,
+            "plan": {k: R.st[k] for k in ("p2p_pairs", "hf_m2l", "src_slots", "trg_slots",
+                                          "m2f_slots", "m2l_entries", "m2l_group_entries",
+                                          "m2l_table_rows", "p2p_items")}}
+    R.close()
     if args.copy_gib > 0:
         nb = int(args.copy_gib * (1 << 30)) // 16
-        src = torch.ones(nb, dtype=torch.complex128, device=dev)
+        src = torch.ones(nb, dtype=torch.complex128, device="cuda:0")
         dst = torch.empty_like(src)
         torch.cuda.synchronize()
         dst.copy_(src)
@@ -304,40 +301,57 @@ def main():
         log(f"copied {nb * 16} bytes device-to-device")
         del src, dst
 
-    gate = None
-    if args.gate_workload != "none":
-        gate = gate_regime(args.gate_workload, args.steps, args.warmup, args.samples,
-                           args.gate_nrhs)
+    scale = None
+    if args.scale_workload != "none":
+        S = Runner(args.scale_workload, 2000, max(1, min(args.steps, 5)), 1)
+        s64, se64, _, sst64 = S.timed(1, "fp64")
+        sroof = S.roofline(sst64, 1, False)
+        sroof["traffic"] = read_traffic("m2l_stored", args.scale_workload)
+        if sroof["traffic"] and "compulsory_bytes_per_launch" in sroof:
+            sroof["traffic_over_compulsory"] = sroof["traffic"] / sroof["compulsory_bytes_per_launch"]
+        S.A.set_precision(dm.MIXED)
+        smx, semx, _, sstmx = S.timed(1, "mixed")
+        scale = {"workload": S.describe(), "setup_s": S.setup_s,
+                 "setup_ms_by_phase": S.setup_ms, "device_gb_mixed": S.A.device_bytes() / 1e9,
+                 "fp64": {"value": S.n / (s64 * 1e-3) / 1e6, "ms_per_step": s64,
+                          "rel_l2_err": se64, "stage_ms": sst64, "roofline": sroof},
+                 "mixed": {"value": S.n / (smx * 1e-3) / 1e6, "ms_per_step": smx,
+                           "rel_l2_err": semx, "stage_ms": sstmx,
+                           "speedup_vs_fp64": s64 / smx},
+                 "note": "order 5 at 128 wavelengths: error 7e-3, above the 1e-6 gate"}
+        S.close()
 
     cpu = None
     if not args.no_cpu:
         t0 = time.time()
-        cpu = cpu_baseline()
+        cpu = cpu_baseline(L=R.L)
         log(f"cpu baseline {cpu} ({time.time() - t0:.1f} s)")
 
+    gate_ok = max(e64, emx, eb) <= 1e-6
     out = {
         "metric": METRIC, "value": value, "unit": "Mpts·RHS/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms64,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": msb,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"{args.workload}: {n:,} points ({geo}), {nl:g} wavelengths, "
-                               f"order {L}, leaf<={ncrit - 1}, 1 RHS, FP64 drop-in",
-                   "n_points": n, "kappa": kappa, "order": L, "ncrit": ncrit, "nrhs": 1,
-                   "parallelism": "single GPU"},
-        "accuracy": {"rel_l2_err": e64, "rel_linf_err": einf64, "targets": args.samples,
+        "config": {"workload": f"{R.describe()}, {args.nrhs} RHS, mixed precision (float "
+                               f"expansions, double phase reduction), stored M2L",
+                   "n_points": n, "kappa": R.kappa, "order": R.L, "ncrit": R.ncrit,
+                   "nrhs": args.nrhs, "parallelism": "single GPU"},
+        "accuracy": {"rel_l2_err": eb, "rel_linf_err": einfb, "targets": args.samples,
+                     "gate_1e-6_met": gate_ok,
                      "reference": "FP64 direct sum (GPU) on sampled targets"},
-        "setup_s": setup_s,
-        "setup_ms_by_phase": setup_ms,
-        "stage_ms": st64,
+        "fp64": {"value": v64, "ms_per_step": ms64, "rel_l2_err": e64, "rel_linf_err": einf64,
+                 "stage_ms": st64, "roofline": roof64},
+        "mixed": {"value": vmx, "ms_per_step": msmx, "rel_l2_err": emx, "rel_linf_err": einfmx,
+                  "speedup_vs_fp64": v64 and vmx / v64, "stage_ms": stmx},
+        "mixed_batched": {"value": value, "nrhs": args.nrhs, "ms_per_step": msb,
+                          "rel_l2_err_first_rhs": eb, "speedup_vs_fp64": value / v64,
+                          "stage_ms": stb, "m2l_variant": variant},
+        "speedup_mixed_batched_vs_fp64": value / v64,
         "roofline": roof,
         "cpu_baseline": cpu,
-        "mixed": mixed,
-        "mixed_batched": batched,
-        "accuracy_gate_1e-6": gate,
-        "m2l_variants_fp64": alt,
-        "plan": {k: st[k] for k in ("p2p_pairs", "hf_m2l", "src_slots", "trg_slots",
-                                      "m2f_slots", "m2l_entries", "m2l_group_entries",
-                                      "m2l_batches", "m2l_batch_rows", "p2p_items")},
+        "headline": head,
+        "scale": scale,
     }
     print(json.dumps(out), flush=True)
 

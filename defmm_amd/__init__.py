@@ -36,7 +36,7 @@ STAT_NAMES = [
     "lf_l2p", "hf_l2l", "hf2lf_l2l", "lf_l2l", "nlvl", "cpt", "max_stage", "ncells_t",
     "ncells_s", "p2p_items", "m2f_slots", "m2l_entries", "m2l_targets", "m2l_groups",
     "m2l_group_entries", "m2l_batches", "m2l_batch_rows", "ms_tree", "ms_walk", "ms_tables",
-    "ms_trans", "ms_m2l", "ms_p2p", "ms_upload",
+    "ms_trans", "ms_m2l", "ms_p2p", "ms_upload", "m2l_table_rows",
 ]
 STAGE_NAMES = ["gather", "p2m", "m2m", "m2f", "m2l_f2l", "l2l", "l2p", "p2p", "scatter"]
 
@@ -85,6 +85,7 @@ def lib():
     L.defmm_plan_set_m2l.argtypes = [P, ctypes.c_int]
     L.defmm_plan_set_m2l_batch_rows.argtypes = [P, ctypes.c_int]
     L.defmm_plan_get_m2l.argtypes = [P, ctypes.POINTER(ctypes.c_int)]
+    L.defmm_plan_set_memory_limit.argtypes = [P, ctypes.c_int64]
     L.defmm_direct.argtypes = [D, ctypes.c_int64, D, D, ctypes.c_int64, ctypes.c_double, D]
     _lib = L
     return L
@@ -271,6 +272,11 @@ class IBFMM_Mat:
         ms = np.zeros(len(STAGE_NAMES))
         _check(lib().defmm_stage_times(self._plan, _dptr(ms), len(ms)))
         return dict(zip(STAGE_NAMES, ms.tolist()))
+
+    def set_memory_limit(self, nbytes):
+        """Device memory budget of later reallocations (0: none)."""
+        self._need_plan()
+        _check(lib().defmm_plan_set_memory_limit(self._plan, int(nbytes)))
 
     def device_bytes(self):
         self._need_plan()

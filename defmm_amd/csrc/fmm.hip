@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/defmm_amd.h"
+#include "grid.hpp"
 #include "kernels.hpp"
 #include "plan.hpp"
 
@@ -23,16 +24,12 @@ namespace dfm {
 
 static thread_local std::string g_err;
 
-// Work-item grids: the dispatch packet counts work-items in 32 bits, so a
-// one-dimensional grid of n blocks x b threads must keep n*b < 2^32.  Lists of
-// millions of items use a 2D grid of at most kGridX blocks per row; a block's
-// item index is blockIdx.y * gridDim.x + blockIdx.x.
-constexpr unsigned kGridX = 32768;
+// Work-item grids (grid.hpp): lists of millions of items use a 2D grid; a
+// block's item index is blockIdx.y * gridDim.x + blockIdx.x.
 __device__ __forceinline__ int itemIndex() { return (int)(blockIdx.y * gridDim.x + blockIdx.x); }
 static inline dim3 itemGrid(int64_t n) {
-  unsigned gx = (unsigned)std::min<int64_t>(n, kGridX);
-  unsigned gy = (unsigned)((n + gx - 1) / gx);
-  return dim3(gx, gy);
+  const Grid2 g = itemGrid2(n);
+  return dim3(g.x, g.y);
 }
 
 // ----------------------------------------------------------------------------
@@ -41,10 +38,12 @@ static inline dim3 itemGrid(int64_t n) {
 
 // fill a complex array with zeros (used instead of hipMemsetAsync so that the
 // sequence can be captured in a graph)
+// Grid-stride over n elements with a capped grid (zeroGrid), so that no
+// dispatch approaches 2^32 work-items whatever n is.
 template <typename R2>
 __global__ void k_zero(R2* a, int64_t n) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     R2 z;
     z.x = 0;
     z.y = 0;
@@ -96,7 +95,7 @@ __global__ __launch_bounds__(128) void k_p2m(const int* __restrict__ leaves, Dev
   typedef typename C2T<R>::T R2;
   constexpr int LL = L * L * L;
   constexpr int NI = (LL + 127) / 128;
-  constexpr int C = 4;  // slots per pass: they share the basis products
+  constexpr int C = L >= 9 ? 2 : 4;  // slots per pass: they share the basis products
   __shared__ R Sw[3][L][64];
   __shared__ R2 w[C][64];
   __shared__ double rel[3][64];
@@ -332,17 +331,23 @@ __global__ __launch_bounds__(128) void k_trans(
 // and forms all L outputs (k_trans reads L values per output).  The entry's
 // 1D matrix is the same for every lane, so it comes through scalar loads
 // rather than LDS.
+// Threads per block: the L^2 lines rounded up to whole waves (one wave up
+// to L = 8, two at L = 9, 10).
+template <int L>
+constexpr int transLinesTB() {
+  return (L * L + 63) / 64 * 64;
+}
 template <typename R, int L>
-__global__ __launch_bounds__(64) void k_trans_lines(
+__global__ __launch_bounds__(transLinesTB<L>()) void k_trans_lines(
     const int* __restrict__ out, const int* __restrict__ beg, const int* __restrict__ in,
     const int* __restrict__ sgn, const int* __restrict__ sigIdx, const double2* __restrict__ edv,
     const typename C2T<R>::T* __restrict__ src, typename C2T<R>::T* __restrict__ dst,
     const typename C2T<R>::T* __restrict__ preTab, const typename C2T<R>::T* __restrict__ postTab,
     const R* __restrict__ mat0, const R* __restrict__ mat1, int accumulate, int nItems) {
   typedef typename C2T<R>::T R2;
-  constexpr int LL = L * L * L, NL = L * L;
-  constexpr int NI = (LL + 63) / 64;
-  static_assert(NL <= 64, "one lane per line needs L*L <= 64");
+  constexpr int LL = L * L * L, NL = L * L, TB = transLinesTB<L>();
+  constexpr int NI = (LL + TB - 1) / TB;
+  static_assert(NL <= TB, "one lane per line");
   __shared__ R2 a[LL], b[LL];
   const int lane = threadIdx.x;
   // XCD-contiguous output order, as in k_trans
@@ -365,7 +370,7 @@ __global__ __launch_bounds__(64) void k_trans_lines(
     const int s = in[e], row = sigIdx[e];
 #pragma unroll
     for (int m = 0; m < NI; m++) {
-      const int I = min(lane + 64 * m, LL - 1);
+      const int I = min(lane + TB * m, LL - 1);
       nv[m] = src[(size_t)s * LL + I];
       np[m] = row >= 0 ? preTab[(size_t)row * LL + I] : c2<R>(1, 0);
     }
@@ -397,7 +402,7 @@ __global__ __launch_bounds__(64) void k_trans_lines(
     __syncthreads();
 #pragma unroll
     for (int m = 0; m < NI; m++) {
-      const int I = lane + 64 * m;
+      const int I = lane + TB * m;
       if (I < LL) a[I] = cmul(nv[m], np[m]);
     }
     if (e + 1 < e1) fetch(e + 1);
@@ -558,13 +563,18 @@ static const int kVariantOfSigma[8] = {0, 1, 0, 2, 3, 0, 4, 5};
 // block of <= 1024 threads covers PP, fewer when 256 threads suffice), TB
 // threads, G target slots per group.  A thread keeps NW F values, G x NW
 // table values and G x NW accumulators in registers.
+// From L = 9 (k_m2l_col) a thread owns one column of the frequency cube: NW
+// = P frequencies, TB = P^2 threads rounded up to whole waves, one target.
 template <int L>
 struct M2LShape {
-  static constexpr int PP = (2 * L - 1) * (2 * L - 1) * (2 * L - 1);
+  static constexpr int P = 2 * L - 1;
+  static constexpr int PP = P * P * P;
+  static constexpr bool kColumns = L >= 9;
   static constexpr int NWmin = (PP + 1023) / 1024;
   static constexpr int NWpref = (PP + 255) / 256 < 3 ? (PP + 255) / 256 : 3;
-  static constexpr int NW = NWmin > NWpref ? NWmin : NWpref;
-  static constexpr int TB = ((PP + NW - 1) / NW + 63) / 64 * 64;
+  static constexpr int NW = kColumns ? P : NWmin > NWpref ? NWmin : NWpref;
+  static constexpr int TB =
+      kColumns ? (P * P + 63) / 64 * 64 : ((PP + NW - 1) / NW + 63) / 64 * 64;
   static constexpr int G = m2lGroupSize(L);
 };
 // With NR right-hand sides per launch a thread holds G x NW x NR
@@ -584,6 +594,7 @@ struct M2LShapeN {
 // occupancy asked for below, and at L = 8 in double without it)
 template <typename R, int L>
 constexpr int m2lMaxNR() {
+  if (M2LShape<L>::kColumns) return sizeof(R) == 8 ? 1 : 2;
   constexpr int NW = M2LShape<L>::NW;
   constexpr int cap = (sizeof(R) == 8 ? 16 : 32) / (M2LShape<L>::G * NW);
   constexpr int w = cap >= 8 ? 8 : cap >= 4 ? 4 : cap >= 2 ? 2 : 1;
@@ -734,6 +745,117 @@ __attribute__((amdgpu_waves_per_eu(m2lWavesPerEU<R, L, NR>(), 8))) void k_m2l(
       }
       Lr[x] = c2<R>(s.x * inv, s.y * inv);
     }
+    }
+  }
+}
+
+// Fused M2L + F2L for orders L >= 9 (same lists and formulas as k_m2l, one
+// target per group): thread t < P^2 owns the column (w1, w2) = t of the
+// frequency cube, all P values of w0, so the first axis of the inverse DFT
+// is summed in registers and the Fourier-domain local expansion J never
+// goes to LDS (J, T1 and T2 in LDS would need 198 KB in double at L = 10;
+// T1 and T2 take 88 KB).  Loads of a column are P^2 elements apart and
+// coalesced across the block's threads.
+template <typename R, int L, int NR>
+__global__ __launch_bounds__(M2LShape<L>::TB) void k_m2l_col(
+    const int* __restrict__ groupBeg, const int* __restrict__ groupTarget,
+    const int* __restrict__ gSrcF, const int* __restrict__ gMask, const int* __restrict__ gCode,
+    const typename C2T<R>::T* __restrict__ F, size_t fStride,
+    const typename C2T<R>::T* __restrict__ U, typename C2T<R>::T* __restrict__ Lout,
+    size_t lStride, const R* __restrict__ twc, const R* __restrict__ tws, int nItems) {
+  typedef typename C2T<R>::T R2;
+  constexpr int P = 2 * L - 1, P2 = P * P, LL = L * L * L, PP = P * P * P;
+  constexpr int TB = M2LShape<L>::TB;
+  static_assert(M2LShape<L>::G == 1, "one target per group");
+  __shared__ R2 T1[L * P2], T2[L * L * P];
+  __shared__ R wc[P], ws[P];
+  const int tid = threadIdx.x;
+  const int b = itemIndex();
+  if (b >= nItems) return;
+  // XCD-contiguous group order, as in k_m2l
+  const int n8 = nItems & ~7;
+  const int grp = b < n8 ? (b & 7) * (n8 >> 3) + (b >> 3) : b;
+  if (tid < P) {
+    wc[tid] = twc[tid];
+    ws[tid] = tws[tid];
+  }
+  // threads beyond the cube's columns load a valid column and store nothing
+  const int col = min(tid, P2 - 1);
+  R2 acc[P][NR];
+#pragma unroll
+  for (int m = 0; m < P; m++)
+#pragma unroll
+    for (int r = 0; r < NR; r++) acc[m][r] = c2<R>(0, 0);
+  const int e0 = groupBeg[grp], e1 = groupBeg[grp + 1];
+  for (int e = e0; e < e1; e++) {
+    const R2* Fr = F + (size_t)gSrcF[e] * PP + col;
+    const R2* Ur = U + tableRow<PP>(gCode[e]) + col;
+    // all of the entry's loads are issued before the first use
+    R2 f[P][NR], u[P];
+#pragma unroll
+    for (int m = 0; m < P; m++)
+#pragma unroll
+      for (int r = 0; r < NR; r++) f[m][r] = Fr[r * fStride + m * P2];
+#pragma unroll
+    for (int m = 0; m < P; m++) u[m] = Ur[m * P2];
+#pragma unroll
+    for (int m = 0; m < P; m++)
+#pragma unroll
+      for (int r = 0; r < NR; r++) {
+        acc[m][r].x += u[m].x * f[m][r].x - u[m].y * f[m][r].y;
+        acc[m][r].y += u[m].x * f[m][r].y + u[m].y * f[m][r].x;
+      }
+  }
+  const int t = groupTarget[grp];
+  if (t < 0) return;  // (uniform over the block)
+  const R inv = (R)(1.0 / (double)PP);
+  __syncthreads();  // twiddles in LDS
+#pragma unroll
+  for (int rr = 0; rr < NR; rr++) {
+    R2* Lr = Lout + rr * lStride + (size_t)t * LL;
+    if (rr > 0) __syncthreads();  // T1 / T2 of the previous right-hand side consumed
+    // axis 0 in registers: T1[k0 * P^2 + col] = sum_w0 J[w0, col] e^{+2 pi i w0 k0 / P}
+    if (tid < P2) {
+#pragma unroll
+      for (int k0 = 0; k0 < L; k0++) {
+        R2 s = c2<R>(0, 0);
+#pragma unroll
+        for (int m = 0; m < P; m++) {
+          const int q = (m * k0) % P;
+          const R2 v = acc[m][rr];
+          s.x += v.x * wc[q] - v.y * ws[q];
+          s.y += v.y * wc[q] + v.x * ws[q];
+        }
+        T1[k0 * P2 + tid] = s;
+      }
+    }
+    __syncthreads();
+    // axis 1: T2[(k0*L + k1)*P + w2], k1 < L
+    for (int x = tid; x < L * L * P; x += TB) {
+      const int v2 = x % P, k1 = (x / P) % L, k0 = x / (L * P);
+      R2 s = c2<R>(0, 0);
+#pragma unroll
+      for (int v1 = 0; v1 < P; v1++) {
+        const int q = (v1 * k1) % P;
+        const R2 v = T1[(k0 * P + v1) * P + v2];
+        s.x += v.x * wc[q] - v.y * ws[q];
+        s.y += v.y * wc[q] + v.x * ws[q];
+      }
+      T2[x] = s;
+    }
+    __syncthreads();
+    // axis 2 and scaling: L[(k0*L + k1)*L + k2]
+    for (int x = tid; x < LL; x += TB) {
+      const int k2 = x % L, r = x / L;
+      R2 s = c2<R>(0, 0);
+#pragma unroll
+      for (int v2 = 0; v2 < P; v2++) {
+        const int q = (v2 * k2) % P;
+        const R2 v = T2[r * P + v2];
+        s.x += v.x * wc[q] - v.y * ws[q];
+        s.y += v.y * wc[q] + v.x * ws[q];
+      }
+      Lr[x] = c2<R>(s.x * inv, s.y * inv);
     }
   }
 }
@@ -912,14 +1034,16 @@ __global__ __launch_bounds__(M2LShape<L>::TB) void k_m2l_fly(const int* __restri
 }
 
 // L = 0 for the target slots that receive no M2L (their L gets only L2L).
+// (grid-stride with a capped grid, as k_zero)
 template <typename R, int L>
 __global__ __launch_bounds__(256) void k_zero_slots(const int* __restrict__ slots, int n,
                                                     typename C2T<R>::T* __restrict__ Lout) {
   constexpr int LL = L * L * L;
-  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i >= (int64_t)n * LL) return;
-  const int s = slots[i / LL];
-  Lout[(size_t)s * LL + (i % LL)] = c2<R>(0, 0);
+  const int64_t total = (int64_t)n * LL, stride = (int64_t)gridDim.x * 256;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < total; i += stride) {
+    const int s = slots[i / LL];
+    Lout[(size_t)s * LL + (i % LL)] = c2<R>(0, 0);
+  }
 }
 
 // L2P (L2P.hpp:121-173 with mapOnTrees.hpp:85-134):
@@ -977,17 +1101,27 @@ __global__ __launch_bounds__(64) void k_l2p(const int* __restrict__ leaves, DevC
         np[m] = hf ? sig2[(size_t)(srow + nd) * LL + I] : c2<R>(1, 0);
       }
     };
+    // double from L = 9: the slot's rows go straight to LDS after the
+    // barrier (holding L^3 / 64 pairs of them in registers spilled)
+    constexpr bool kRegs = sizeof(R) == 4 || L <= 8;
     if (kAhead && ns > 0) fetch(0);
     for (int k = 0; k < ns; k++) {
-      if (!kAhead) fetch(k);
-      const int d = nd;
+      if (!kAhead && kRegs) fetch(k);
+      int d = nd;
       __syncthreads();
+      if (kRegs) {
 #pragma unroll
-      for (int m = 0; m < NI; m++) {
-        const int I = tid + 64 * m;
-        if (I < LL) buf[I] = cmul(nv[m], np[m]);
+        for (int m = 0; m < NI; m++) {
+          const int I = tid + 64 * m;
+          if (I < LL) buf[I] = cmul(nv[m], np[m]);
+        }
+        if (kAhead && k + 1 < ns) fetch(k + 1);
+      } else {
+        d = hf ? T.slotDir[s0 + k] : 0;
+        for (int I = tid; I < LL; I += 64)
+          buf[I] = cmul(Lst[(size_t)(s0 + k) * LL + I],
+                        hf ? sig2[(size_t)(srow + d) * LL + I] : c2<R>(1, 0));
       }
-      if (kAhead && k + 1 < ns) fetch(k + 1);
       __syncthreads();
       R2 s = c2<R>(0, 0);
 #pragma unroll
@@ -1345,7 +1479,8 @@ static void launchTrans(const DevTrans& t, const typename C2T<R>::T* src,
                         const typename C2T<R>::T* post, const R* m0, const R* m1, int accumulate,
                         hipStream_t s) {
   if (sizeof(R) == 4 || L >= 6)
-    hipLaunchKernelGGL((k_trans_lines<R, L>), itemGrid(t.n), dim3(64), 0, s, t.out, t.beg, t.in,
+    hipLaunchKernelGGL((k_trans_lines<R, L>), itemGrid(t.n), dim3(transLinesTB<L>()), 0, s,
+                       t.out, t.beg, t.in,
                        t.sgn, t.sigIdx, t.edv, src, dst, pre, post, m0, m1, accumulate, t.n);
   else
     hipLaunchKernelGGL((k_trans<R, L>), itemGrid(t.n), dim3(128), 0, s, t.out, t.beg, t.in,
@@ -1417,26 +1552,61 @@ struct Exec {
   std::vector<void*> typedMem;
   int64_t typedBytes = 0;
   bool typedPhase = false;  // allocations go to typedMem
+  // The typed stores (expansions, tables, charges, potentials) are complete:
+  // cleared before they are released, set only after a successful upload.
+  // Executions refuse to run while it is false.
+  bool valid = false;
+  // Completion of the last execution and the stream it ran on: an execution
+  // on another stream waits for it (every execution shares the stores), and
+  // the setters and destroy wait for it before releasing memory.
+  hipEvent_t done = nullptr;
+  hipStream_t lastStream = nullptr;
+  bool doneRecorded = false;
 
   void freeTyped() {
+    valid = false;
     for (void* m : typedMem) (void)hipFree(m);
     typedMem.clear();
     bytes -= typedBytes;
     typedBytes = 0;
+    sig = sig2 = U = consts = nullptr;
+    ML = F = Lb = q = p = nullptr;
+    gSrc = bRows = nullptr;
+    partial = qio = pio = nullptr;
+  }
+  // wait until the last execution, on whatever stream, has finished
+  int quiesce() {
+    if (hipSetDevice(device) != hipSuccess) return ERR_HIP;
+    if (doneRecorded && hipEventSynchronize(done) != hipSuccess) return ERR_HIP;
+    if (own && hipStreamSynchronize(own) != hipSuccess) return ERR_HIP;
+    return OK;
   }
   ~Exec() {
+    (void)quiesce();
     for (void* m : typedMem) (void)hipFree(m);
     for (void* m : mem) (void)hipFree(m);
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
+    if (done) (void)hipEventDestroy(done);
     if (own) (void)hipStreamDestroy(own);
   }
 
+  // device memory budget (bytes, 0: none), checked by every allocation
+  int64_t memLimit = 0;
+  int overBudget(size_t nbytes) {
+    if (memLimit > 0 && bytes + (int64_t)nbytes > memLimit) {
+      g_err = "the plan's device memory limit (" + std::to_string(memLimit) +
+              " bytes) would be exceeded";
+      return ERR_MEMORY;
+    }
+    return OK;
+  }
   template <class X>
   int up(X*& dst, const X* src, size_t n) {
     dst = nullptr;
     if (n == 0) n = 1;
     void* d = nullptr;
+    if (overBudget(n * sizeof(X))) return ERR_MEMORY;
     HIPCK(hipMalloc(&d, n * sizeof(X)));
     (typedPhase ? typedMem : mem).push_back(d);
     bytes += (int64_t)(n * sizeof(X));
@@ -1452,6 +1622,7 @@ struct Exec {
   int alloc(void*& dst, size_t nbytes) {
     void* d = nullptr;
     if (nbytes == 0) nbytes = 16;
+    if (overBudget(nbytes)) return ERR_MEMORY;
     HIPCK(hipMalloc(&d, nbytes));
     (typedPhase ? typedMem : mem).push_back(d);
     bytes += (int64_t)nbytes;
@@ -1675,6 +1846,8 @@ static int chooseM2LMode(Exec& X) {
     g_err = "hipMemGetInfo failed";
     return ERR_HIP;
   }
+  if (X.memLimit > 0)
+    freeB = std::min(freeB, (size_t)std::max<int64_t>(0, X.memLimit - X.bytes));
   const size_t r2 = X.prec == DEFMM_FP64 ? 16 : 8;
   const size_t storedB =
       (size_t)X.nr * (((size_t)X.nm2f * X.PP + X.nMLslots * X.LL) * r2 + ((size_t)X.ns + X.nt) * (r2 + 16) +
@@ -1688,6 +1861,34 @@ static int uploadPrecision(Exec& X) {
   X.freeTyped();
   int rc = X.prec == DEFMM_FP64 ? uploadTyped<double>(X) : uploadTyped<float>(X);
   X.typedPhase = false;
+  if (rc != OK) {
+    X.freeTyped();  // a partial upload leaves nothing behind
+    return rc;
+  }
+  X.valid = true;
+  return OK;
+}
+
+// Re-upload the typed stores after a change of width, precision or M2L
+// variant.  On failure the previous settings are restored and uploaded again,
+// so that the plan stays usable; if that fails too, the plan stays invalid
+// and executions return DEFMM_ESTATE.
+static int reupload(Exec& X, int oldNr, int oldPrec, int oldMode, bool oldAuto, int oldRows) {
+  X.evRecorded = false;
+  X.freeTyped();  // the variant is chosen by the memory free without them
+  int rc = chooseM2LMode(X);
+  if (rc == OK) rc = uploadPrecision(X);
+  if (rc == OK) return OK;
+  const std::string why = g_err;
+  X.nr = oldNr;
+  X.prec = oldPrec;
+  X.m2lMode = oldMode;
+  X.m2lAuto = oldAuto;
+  X.batchRows = oldRows;
+  if (uploadPrecision(X) == OK)
+    g_err = why + " (the plan keeps its previous settings)";
+  else
+    g_err = why + " (the previous settings could not be restored: the plan is unusable)";
   return rc;
 }
 
@@ -1799,6 +2000,7 @@ static int upload(Exec& X, Plan& pl) {
         dc[e * G + g] = rowOf[key] << 3;
       }
     if (X.hRowKey.empty()) X.hRowKey.push_back(0);
+    X.st.m2l_table_rows = (int64_t)X.hRowKey.size();
     CKR(X.upv(X.gCode, dc));
   }
   X.hGSrcF.swap(pl.m2lGSrcF);
@@ -1881,9 +2083,14 @@ static void launchM2L(Exec& X, hipStream_t s, int g0, int ng, const typename C2T
                       typename C2T<R>::T* Lb, const OpConsts<R>& oc) {
   typedef typename C2T<R>::T R2;
   constexpr int G = M2LShapeN<L, NR>::G;
-  hipLaunchKernelGGL((k_m2l<R, L, NR>), itemGrid(ng), dim3(M2LShapeN<L, NR>::TB), 0, s,
-                     X.gBeg + g0, X.gTarget + (size_t)g0 * G, X.gSrc, X.gMask, X.gCode, F,
-                     X.fStride, (const R2*)X.U, Lb, X.lStride, oc.twc, oc.tws, ng);
+  if constexpr (M2LShape<L>::kColumns)
+    hipLaunchKernelGGL((k_m2l_col<R, L, NR>), itemGrid(ng), dim3(M2LShape<L>::TB), 0, s,
+                       X.gBeg + g0, X.gTarget + (size_t)g0 * G, X.gSrc, X.gMask, X.gCode, F,
+                       X.fStride, (const R2*)X.U, Lb, X.lStride, oc.twc, oc.tws, ng);
+  else
+    hipLaunchKernelGGL((k_m2l<R, L, NR>), itemGrid(ng), dim3(M2LShapeN<L, NR>::TB), 0, s,
+                       X.gBeg + g0, X.gTarget + (size_t)g0 * G, X.gSrc, X.gMask, X.gCode, F,
+                       X.fStride, (const R2*)X.U, Lb, X.lStride, oc.twc, oc.tws, ng);
 }
 
 template <typename R, int L>
@@ -1950,8 +2157,8 @@ static int runTyped(Exec& X, const double2* qin, double2* pout, int parts, int n
   // (a failed record surfaces at the launch check below or in stage_times)
   auto mark = [&](int k) { (void)hipEventRecord(X.ev[k], s); };
   mark(0);
-  hipLaunchKernelGGL(k_zero<R2>, dim3(nblk((int64_t)(nrhs * nt), 256)), dim3(256), 0, s,
-                     (R2*)X.p, (int64_t)(nrhs * nt));
+  hipLaunchKernelGGL(k_zero<R2>, dim3(zeroGrid((int64_t)nrhs * (int64_t)nt, 256)), dim3(256), 0,
+                     s, (R2*)X.p, (int64_t)nrhs * (int64_t)nt);
   for (int r = 0; r < nrhs; r++)
     hipLaunchKernelGGL(k_gather<R>, dim3(nblk(X.ns, 256)), dim3(256), 0, s, qin + r * ns,
                        X.sperm, qr(r), (int)X.ns);
@@ -1981,10 +2188,15 @@ static int runTyped(Exec& X, const double2* qin, double2* pout, int parts, int n
       // L (aliasing M) is written for every target slot from here on
       for (int r0 = 0; r0 < nrhs;) r0 += m2lChunk<R, L>(X, s, 0, X.nGroups, r0, nrhs - r0, oc);
     } else if (X.m2lMode == DEFMM_M2L_ON_THE_FLY) {
-      for (int r = 0; r < nrhs; r++)
-        hipLaunchKernelGGL((k_m2l_fly<R, L>), itemGrid(X.nGroups), dim3(M2LShape<L>::TB), 0, s,
-                           X.gBeg, X.gTarget, X.gSrc, X.gMask, X.gCode, (const R2*)Mr(r),
-                           (const R2*)X.U, Lr(r), oc.twc, oc.tws, X.nGroups);
+      if constexpr (L <= 8) {
+        for (int r = 0; r < nrhs; r++)
+          hipLaunchKernelGGL((k_m2l_fly<R, L>), itemGrid(X.nGroups), dim3(M2LShape<L>::TB), 0,
+                             s, X.gBeg, X.gTarget, X.gSrc, X.gMask, X.gCode, (const R2*)Mr(r),
+                             (const R2*)X.U, Lr(r), oc.twc, oc.tws, X.nGroups);
+      } else {
+        g_err = "the on-the-fly M2L variant supports orders up to 8";
+        return ERR_ARG;
+      }
     } else {
       // per batch: its F rows into the workspace, then its groups' M2L
       // (stream order separates a batch's M2L from the next batch's M2F)
@@ -2003,7 +2215,7 @@ static int runTyped(Exec& X, const double2* qin, double2* pout, int parts, int n
   mark(5);
   if (far && X.nM2LZero)
     for (int r = 0; r < nrhs; r++)
-      hipLaunchKernelGGL((k_zero_slots<R, L>), dim3(nblk((int64_t)X.nM2LZero * X.LL, 256)),
+      hipLaunchKernelGGL((k_zero_slots<R, L>), dim3(zeroGrid((int64_t)X.nM2LZero * X.LL, 256)),
                          dim3(256), 0, s, X.m2lZero, X.nM2LZero, Lr(r));
   if (far)
     for (int r = 0; r < nrhs; r++)
@@ -2048,14 +2260,23 @@ static int runR(Exec& X, const double2* qin, double2* pout, int parts, int nrhs,
     case 6: return runTyped<R, 6>(X, qin, pout, parts, nrhs, s);
     case 7: return runTyped<R, 7>(X, qin, pout, parts, nrhs, s);
     case 8: return runTyped<R, 8>(X, qin, pout, parts, nrhs, s);
+    case 9: return runTyped<R, 9>(X, qin, pout, parts, nrhs, s);
+    case 10: return runTyped<R, 10>(X, qin, pout, parts, nrhs, s);
   }
-  g_err = "interpolation order outside 2..8";
+  g_err = "interpolation order outside 2..10";
   return ERR_ARG;
 }
 
 // nrhs right-hand sides in chunks of the plan's width
 static int run(Exec& X, const double2* qin, double2* pout, int parts, int nrhs, hipStream_t s) {
+  if (!X.valid) {
+    g_err = "the plan's device stores are not allocated (a failed reallocation left it unusable)";
+    return ERR_STATE;
+  }
   HIPCK(hipSetDevice(X.device));
+  // every execution uses the plan's stores: one on another stream than the
+  // last starts after it
+  if (X.doneRecorded && s != X.lastStream) HIPCK(hipStreamWaitEvent(s, X.done, 0));
   for (int r0 = 0; r0 < nrhs; r0 += X.nr) {
     const int n = std::min(X.nr, nrhs - r0);
     const double2* q = qin + (size_t)r0 * X.ns;
@@ -2064,6 +2285,9 @@ static int run(Exec& X, const double2* qin, double2* pout, int parts, int nrhs, 
                                   : runR<float>(X, q, p, parts, n, s);
     if (rc) return rc;
   }
+  HIPCK(hipEventRecord(X.done, s));
+  X.lastStream = s;
+  X.doneRecorded = true;
   return OK;
 }
 
@@ -2104,7 +2328,7 @@ int defmm_plan_create(const double* src_xyz, int64_t n_src, const double* trg_xy
   if (!out) { g_err = "null output pointer"; return DEFMM_EARG; }
   *out = nullptr;
   if (precision != DEFMM_FP64 && precision != DEFMM_MIXED) { g_err = "bad precision"; return DEFMM_EARG; }
-  if (order < 2 || order > dfm::LMAX) { g_err = "order must be in 2..8"; return DEFMM_EARG; }
+  if (order < 2 || order > dfm::LMAX) { g_err = "order must be in 2..10"; return DEFMM_EARG; }
   if (lvmax < -1 || (lvmax < 0 && ncrit < 1)) { g_err = "bad ncrit/lvmax"; return DEFMM_EARG; }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) { g_err = "no HIP device"; return DEFMM_ENODEVICE; }
@@ -2132,6 +2356,8 @@ int defmm_plan_create(const double* src_xyz, int64_t n_src, const double* trg_xy
   delete pl;
   if (rc == dfm::OK) {
     if (hipStreamCreate(&h->x.own) != hipSuccess) rc = DEFMM_EHIP;
+    if (rc == dfm::OK && hipEventCreateWithFlags(&h->x.done, hipEventDisableTiming) != hipSuccess)
+      rc = DEFMM_EHIP;
     for (auto& e : h->x.ev)
       if (rc == dfm::OK && hipEventCreate(&e) != hipSuccess) rc = DEFMM_EHIP;
   }
@@ -2160,7 +2386,18 @@ static int execHost(defmm_plan plan, const double* q, double* p, int parts, int 
   if (parts < 1 || parts > 3) { g_err = "parts must be 1, 2 or 3"; return DEFMM_EARG; }
   if (nrhs < 1) { g_err = "nrhs must be positive"; return DEFMM_EARG; }
   Exec& X = plan->x;
+  if (!X.valid) {
+    g_err = "the plan's device stores are not allocated (a failed reallocation left it unusable)";
+    return DEFMM_ESTATE;
+  }
   if (hipSetDevice(X.device) != hipSuccess) { g_err = "hipSetDevice failed"; return DEFMM_EHIP; }
+  // the staging buffers are written on the plan's own stream: after the last
+  // execution, whatever stream it ran on
+  if (X.doneRecorded && X.lastStream != X.own &&
+      hipStreamWaitEvent(X.own, X.done, 0) != hipSuccess) {
+    g_err = "hipStreamWaitEvent failed";
+    return DEFMM_EHIP;
+  }
   for (int r0 = 0; r0 < nrhs; r0 += X.nr) {
     const int n = std::min(X.nr, nrhs - r0);
     if (hipMemcpyAsync(X.qio, q + 2 * (size_t)r0 * X.ns, n * X.ns * sizeof(double2),
@@ -2196,16 +2433,14 @@ int defmm_plan_set_rhs(defmm_plan plan, int nrhs) {
   if (!plan) { g_err = "null plan"; return DEFMM_EARG; }
   if (nrhs < 1 || nrhs > 64) { g_err = "nrhs must be in 1..64"; return DEFMM_EARG; }
   Exec& X = plan->x;
-  if (nrhs == X.nr) return DEFMM_OK;
-  if (hipSetDevice(X.device) != hipSuccess || hipStreamSynchronize(X.own) != hipSuccess) {
+  if (nrhs == X.nr && X.valid) return DEFMM_OK;
+  if (X.quiesce() != dfm::OK) {
     g_err = "device synchronisation failed";
     return DEFMM_EHIP;
   }
+  const int oldNr = X.nr;
   X.nr = nrhs;
-  X.evRecorded = false;
-  X.freeTyped();
-  int rc = dfm::chooseM2LMode(X);
-  return rc ? rc : dfm::uploadPrecision(X);
+  return dfm::reupload(X, oldNr, X.prec, X.m2lMode, X.m2lAuto, X.batchRows);
 }
 
 int defmm_plan_stats(defmm_plan plan, int64_t* out, int n) {
@@ -2219,7 +2454,7 @@ int defmm_plan_stats(defmm_plan plan, int64_t* out, int n) {
                              s.m2l_entries, s.m2l_targets, s.m2l_groups,
                              s.m2l_group_entries, s.m2l_batches, s.m2l_batch_rows,
                              s.ms_tree, s.ms_walk, s.ms_tables, s.ms_trans, s.ms_m2l,
-                             s.ms_p2p, s.ms_upload};
+                             s.ms_p2p, s.ms_upload, s.m2l_table_rows};
   for (int i = 0; i < n && i < DEFMM_NSTATS; i++) out[i] = v[i];
   return DEFMM_OK;
 }
@@ -2242,14 +2477,14 @@ int defmm_plan_set_precision(defmm_plan plan, int precision) {
   if (!plan) { g_err = "null plan"; return DEFMM_EARG; }
   if (precision != DEFMM_FP64 && precision != DEFMM_MIXED) { g_err = "bad precision"; return DEFMM_EARG; }
   Exec& X = plan->x;
-  if (precision == X.prec) return DEFMM_OK;
-  if (hipSetDevice(X.device) != hipSuccess || hipStreamSynchronize(X.own) != hipSuccess) {
+  if (precision == X.prec && X.valid) return DEFMM_OK;
+  if (X.quiesce() != dfm::OK) {
     g_err = "device synchronisation failed";
     return DEFMM_EHIP;
   }
+  const int oldPrec = X.prec;
   X.prec = precision;
-  X.evRecorded = false;
-  return dfm::uploadPrecision(X);
+  return dfm::reupload(X, X.nr, oldPrec, X.m2lMode, X.m2lAuto, X.batchRows);
 }
 
 int defmm_plan_set_m2l(defmm_plan plan, int mode) {
@@ -2259,15 +2494,23 @@ int defmm_plan_set_m2l(defmm_plan plan, int mode) {
     return DEFMM_EARG;
   }
   Exec& X = plan->x;
-  if (mode == X.m2lMode) return DEFMM_OK;
-  if (hipSetDevice(X.device) != hipSuccess || hipStreamSynchronize(X.own) != hipSuccess) {
+  if (mode == DEFMM_M2L_ON_THE_FLY && X.L > 8) {
+    g_err = "the on-the-fly M2L variant supports orders up to 8";
+    return DEFMM_EARG;
+  }
+  if (mode == X.m2lMode && X.valid) {
+    X.m2lAuto = false;
+    return DEFMM_OK;
+  }
+  if (X.quiesce() != dfm::OK) {
     g_err = "device synchronisation failed";
     return DEFMM_EHIP;
   }
+  const int oldMode = X.m2lMode;
+  const bool oldAuto = X.m2lAuto;
   X.m2lMode = mode;
   X.m2lAuto = false;
-  X.evRecorded = false;
-  return dfm::uploadPrecision(X);
+  return dfm::reupload(X, X.nr, X.prec, oldMode, oldAuto, X.batchRows);
 }
 
 int defmm_plan_get_m2l(defmm_plan plan, int* mode) {
@@ -2280,14 +2523,21 @@ int defmm_plan_set_m2l_batch_rows(defmm_plan plan, int rows) {
   if (!plan) { g_err = "null plan"; return DEFMM_EARG; }
   if (rows < 1) { g_err = "batch rows must be positive"; return DEFMM_EARG; }
   Exec& X = plan->x;
+  const int oldRows = X.batchRows;
   X.batchRows = rows;
-  if (X.m2lMode != DEFMM_M2L_BATCHED) return DEFMM_OK;
-  if (hipSetDevice(X.device) != hipSuccess || hipStreamSynchronize(X.own) != hipSuccess) {
+  if (X.m2lMode != DEFMM_M2L_BATCHED && X.valid) return DEFMM_OK;
+  if (X.quiesce() != dfm::OK) {
     g_err = "device synchronisation failed";
     return DEFMM_EHIP;
   }
-  X.evRecorded = false;
-  return dfm::uploadPrecision(X);
+  return dfm::reupload(X, X.nr, X.prec, X.m2lMode, X.m2lAuto, oldRows);
+}
+
+int defmm_plan_set_memory_limit(defmm_plan plan, int64_t bytes) {
+  if (!plan) { g_err = "null plan"; return DEFMM_EARG; }
+  if (bytes < 0) { g_err = "memory limit must be >= 0"; return DEFMM_EARG; }
+  plan->x.memLimit = bytes;
+  return DEFMM_OK;
 }
 
 int defmm_plan_device_bytes(defmm_plan plan, int64_t* bytes) {
@@ -2336,7 +2586,7 @@ int defmm_plan_build_stats(const double* src_xyz, int64_t n_src, const double* t
                            int64_t n_trg, int order, int ncrit, double kappa, int lvmax,
                            int64_t* out, int n) {
   if (!out) { g_err = "null output pointer"; return DEFMM_EARG; }
-  if (order < 2 || order > dfm::LMAX) { g_err = "order must be in 2..8"; return DEFMM_EARG; }
+  if (order < 2 || order > dfm::LMAX) { g_err = "order must be in 2..10"; return DEFMM_EARG; }
   if (lvmax < -1 || (lvmax < 0 && ncrit < 1)) { g_err = "bad ncrit/lvmax"; return DEFMM_EARG; }
   dfm::Plan* pl = new dfm::Plan();
   std::string msg;
@@ -2355,8 +2605,7 @@ int defmm_plan_build_stats(const double* src_xyz, int64_t n_src, const double* t
 
 int defmm_plan_destroy(defmm_plan plan) {
   if (!plan) return DEFMM_OK;
-  (void)hipSetDevice(plan->x.device);
-  (void)hipStreamSynchronize(plan->x.own);
+  (void)plan->x.quiesce();
   delete plan;
   return DEFMM_OK;
 }

@@ -11,7 +11,7 @@
 
 namespace dfm {
 
-constexpr int LMAX = 8;  // largest interpolation order supported on device
+constexpr int LMAX = 10;  // largest interpolation order supported on device
 
 template <typename R> struct C2T;
 template <> struct C2T<double> { typedef double2 T; };

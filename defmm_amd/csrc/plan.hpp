@@ -21,9 +21,10 @@ typedef std::complex<double> cplx;
 // frequencies, so that the FP64 kernel fits its registers; 3 at L = 7, where
 // the float kernel with two right-hand sides then fits two 768-thread blocks
 // per CU: 1M points / 16 wavelengths, 4 right-hand sides 182.8 -> 159.8 ms,
-// FP64 89.1 -> 91.4 ms).
+// FP64 89.1 -> 91.4 ms; 1 from L = 9, where a thread owns a whole column of
+// 2L - 1 frequencies).
 constexpr int kM2LGroupMax = 8;
-constexpr int m2lGroupSize(int L) { return L >= 8 ? 2 : L == 7 ? 3 : 4; }
+constexpr int m2lGroupSize(int L) { return L >= 9 ? 1 : L == 8 ? 2 : L == 7 ? 3 : 4; }
 
 // Error codes returned through the C ABI.
 enum {
@@ -74,6 +75,7 @@ struct Stats {
   // phase rows, M2M/L2L lists, M2L lists and groups, P2P lists
   int64_t ms_tree = 0, ms_walk = 0, ms_tables = 0, ms_trans = 0, ms_m2l = 0, ms_p2p = 0;
   int64_t ms_upload = 0;  // device upload (set by the executor)
+  int64_t m2l_table_rows = 0;  // rows of the materialised M2L table store (set at upload)
 };
 
 // A per-level translation list (M2M or L2L): for every output slot, the

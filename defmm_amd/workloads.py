@@ -119,11 +119,17 @@ def kappa_for(n_lambda, *clouds):
 CONFIGS = {
     # name: (geometry, n, n_lambda, L, Ncrit)  -- Ncrit = leaf size + 1
     "cube20k_32l": ("cube_edges", 20000, 32, 4, 65),
+    # the same cube at order 8, the lowest order whose relative error against
+    # the FP64 direct sum is <= 1e-6 there (7.2e-7; order 7 gives 5.8e-6)
+    "cube20k_32l_L8": ("cube_edges", 20000, 32, 8, 65),
+    "cube20k_32l_L9": ("cube_edges", 20000, 32, 9, 65),
     "sphere1m_128l": ("sphere", 1000000, 128, 5, 65),
     # same sphere at 32 wavelengths with order 7: the relative-error <= 1e-6 regime
     "sphere1m_32l_L7": ("sphere", 1000000, 32, 7, 65),
     "sphere1m_32l_L8": ("sphere", 1000000, 32, 8, 65),
     "sphere1m_16l_L7": ("sphere", 1000000, 16, 7, 65),
+    "ellipsoid1m_128l": ("ellipsoid", 1000000, 128, 5, 65),
+    "ellipsoid2m_256l_L6": ("ellipsoid", 2000000, 256, 6, 97),
 }
 
 

@@ -11,6 +11,10 @@
  * Only plain pointers and sizes cross this boundary.  Every entry point
  * returns 0 on success or a DEFMM_E* code; defmm_last_error() gives a message
  * (thread-local).  A plan is thread-compatible (one caller at a time).
+ * Executions of one plan share its device stores, so they are serialised
+ * whatever streams they are enqueued on: an execution on another stream than
+ * the previous one waits (on the device) for the previous one to finish, and
+ * the reallocating setters and defmm_plan_destroy wait on the host.
  */
 #ifndef DEFMM_AMD_H
 #define DEFMM_AMD_H
@@ -168,6 +172,17 @@ int defmm_plan_get_m2l(defmm_plan plan, int* mode);
  */
 int defmm_plan_set_m2l_batch_rows(defmm_plan plan, int rows);
 
+/*
+ * Device memory budget of a plan in bytes (0 = none, the default).  Later
+ * reallocations (defmm_plan_set_rhs, _set_precision, _set_m2l,
+ * _set_m2l_batch_rows) that would exceed it fail with DEFMM_EMEMORY, and the
+ * automatic M2L variant choice counts only the memory left under it.  The
+ * current allocation is not shrunk.  A reallocation that fails for any reason
+ * restores the previous settings; if even those cannot be allocated again,
+ * the plan stays unusable and executions return DEFMM_ESTATE.
+ */
+int defmm_plan_set_memory_limit(defmm_plan plan, int64_t bytes);
+
 /* Bytes of device memory held by the plan. */
 int defmm_plan_device_bytes(defmm_plan plan, int64_t* bytes);
 
@@ -191,6 +206,9 @@ enum {
      rows, M2M/L2L lists, M2L lists, P2P lists, device upload */
   DEFMM_STAT_MS_TREE, DEFMM_STAT_MS_WALK, DEFMM_STAT_MS_TABLES, DEFMM_STAT_MS_TRANS,
   DEFMM_STAT_MS_M2L, DEFMM_STAT_MS_P2P, DEFMM_STAT_MS_UPLOAD,
+  /* rows of the M2L kernel-table store on the device (one per table, axis
+     order and sign flips in use) */
+  DEFMM_STAT_M2L_TABLE_ROWS,
   DEFMM_NSTATS
 };
 

@@ -1,0 +1,178 @@
+"""GPU tests of the C ABI's execution paths and plan life cycle:
+
+  - device-pointer executions (single and batched), which the benchmark
+    times, equal the host-array executions;
+  - switching the arithmetic of an existing plan FP64 -> MIXED -> FP64 gives
+    the results of fresh plans of each arithmetic;
+  - mixed precision at order 8 (float M2L kernels with 4 right-hand sides per
+    launch) with 5 right-hand sides against FP64;
+  - a reallocation that cannot be satisfied (over the plan's memory limit)
+    returns an error and leaves the plan usable with its previous settings;
+    when even those cannot be restored, executions return an error instead
+    of running;
+  - executions on two streams are serialised by the plan.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _diff(a, b):
+    d = np.abs(a - b)
+    return np.sqrt((d ** 2).sum() / (np.abs(b) ** 2).sum()), d.max() / np.abs(b).max()
+
+
+def _cloud(n=6000, seed=12):
+    from defmm_amd import workloads as wl
+    pts = wl.ellipsoid_clusters(n, seed=seed)
+    return pts, wl.kappa_for(16, pts)
+
+
+def _plan(dm, pts, L, ncrit, kappa, prec):
+    A = dm.IBFMM_Mat(precision=prec)
+    A.setSourceParticles(pts)
+    A.setTargetParticles(pts)
+    A.prcmpt(L, ncrit, kappa)
+    return A
+
+
+def test_device_execute_matches_host():
+    import torch
+    import defmm_amd as dm
+    from defmm_amd import workloads as wl
+    pts, kappa = _cloud()
+    n, nr = len(pts), 3
+    Q = np.stack([wl.charges(n, seed=500 + r) for r in range(nr)])
+    dev = torch.device("cuda:0")
+    for prec in (dm.FP64, dm.MIXED):
+        _say(f"case: device-pointer execution vs host arrays, ellipsoid N={n} L=5, precision {prec}")
+        A = _plan(dm, pts, 5, 65, kappa, prec)
+        ph = A.apply(Q[0])
+        qd = torch.from_numpy(Q[0]).to(dev)
+        pd = torch.full((n,), 7 + 7j, dtype=torch.complex128, device=dev)
+        s = torch.cuda.current_stream(dev)
+        A.apply_device(qd.data_ptr(), pd.data_ptr(), s.cuda_stream)
+        torch.cuda.synchronize()
+        assert np.array_equal(pd.cpu().numpy(), ph)
+        A.set_rhs(nr)
+        Ph = A.apply_batch(Q)
+        Qd = torch.from_numpy(Q).to(dev)
+        Pd = torch.zeros((nr, n), dtype=torch.complex128, device=dev)
+        A.apply_batch_device(Qd.data_ptr(), Pd.data_ptr(), nr, s.cuda_stream)
+        torch.cuda.synchronize()
+        A.destroy()
+        assert np.array_equal(Pd.cpu().numpy(), Ph)
+        l2, _ = _diff(Ph[0], ph)
+        _say(f"  batched first right-hand side vs single rel L2 {l2:.3e}")
+        assert l2 <= (1e-13 if prec == dm.FP64 else 1e-6)
+
+
+def test_set_precision_roundtrip_equals_fresh_plans():
+    import defmm_amd as dm
+    from defmm_amd import workloads as wl
+    pts, kappa = _cloud()
+    q = wl.charges(len(pts), seed=41)
+    _say("case: FP64 -> MIXED -> FP64 on one plan vs fresh plans, L=6")
+    fresh = {}
+    for prec in (dm.FP64, dm.MIXED):
+        B = _plan(dm, pts, 6, 65, kappa, prec)
+        fresh[prec] = B.apply(q)
+        B.destroy()
+    A = _plan(dm, pts, 6, 65, kappa, dm.FP64)
+    p0 = A.apply(q)
+    A.set_precision(dm.MIXED)
+    p1 = A.apply(q)
+    A.set_precision(dm.FP64)
+    p2 = A.apply(q)
+    A.destroy()
+    assert np.array_equal(p0, fresh[dm.FP64])
+    assert np.array_equal(p1, fresh[dm.MIXED])
+    assert np.array_equal(p2, fresh[dm.FP64])
+    l2, _ = _diff(p1, p0)
+    _say(f"  mixed vs fp64 rel L2 {l2:.3e}")
+    assert l2 <= 3e-6
+
+
+def test_mixed_order8_five_rhs():
+    import defmm_amd as dm
+    from defmm_amd import workloads as wl
+    pts, kappa = _cloud(5000, seed=13)
+    n, nr = len(pts), 5
+    Q = np.stack([wl.charges(n, seed=600 + r) for r in range(nr)])
+    _say(f"case: order 8, {nr} right-hand sides (float M2L 4 + 1 per launch) vs FP64")
+    A = _plan(dm, pts, 8, 65, kappa, dm.FP64)
+    st = A.stats()
+    assert st["hf_m2l"] + st["lf_m2l"] > 0
+    ref = np.stack([A.apply(Q[r]) for r in range(nr)])
+    A.set_precision(dm.MIXED)
+    single = np.stack([A.apply(Q[r]) for r in range(nr)])
+    A.set_rhs(nr)
+    P = A.apply_batch(Q)
+    A.destroy()
+    l2s, _ = _diff(P, single)
+    l2r = max(_diff(P[r], ref[r])[0] for r in range(nr))
+    _say(f"  batch vs single (mixed) rel L2 {l2s:.3e}; worst mixed vs FP64 {l2r:.3e}")
+    assert l2s <= 1e-6
+    assert l2r <= 3e-6
+
+
+def test_failed_reallocation_keeps_or_invalidates_plan():
+    import defmm_amd as dm
+    from defmm_amd import workloads as wl
+    pts, kappa = _cloud(4000, seed=14)
+    q = wl.charges(len(pts), seed=42)
+    _say("case: reallocation over the plan's memory limit")
+    A = _plan(dm, pts, 5, 65, kappa, dm.FP64)
+    p0 = A.apply(q)
+    used = A.device_bytes()
+    A.set_memory_limit(used + 4096)
+    with pytest.raises(dm.DefmmError, match="out of memory"):
+        A.set_rhs(16)
+    # previous settings restored: same width, same results
+    assert A.device_bytes() == used
+    assert np.array_equal(A.apply(q), p0)
+    # a limit below what the plan already holds: the restore fails as well,
+    # and executions refuse to run
+    A.set_memory_limit(1)
+    with pytest.raises(dm.DefmmError, match="out of memory"):
+        A.set_precision(dm.MIXED)
+    with pytest.raises(dm.DefmmError, match="call out of sequence"):
+        A.apply(q)
+    # lifting the limit and asking again makes it usable
+    A.set_memory_limit(0)
+    A.set_precision(dm.FP64)
+    assert np.array_equal(A.apply(q), p0)
+    A.destroy()
+
+
+def test_executions_on_two_streams_are_serialised():
+    import torch
+    import defmm_amd as dm
+    from defmm_amd import workloads as wl
+    pts, kappa = _cloud()
+    n = len(pts)
+    q1, q2 = wl.charges(n, seed=43), wl.charges(n, seed=44)
+    _say("case: executions of one plan on two streams")
+    A = _plan(dm, pts, 5, 65, kappa, dm.FP64)
+    ref1, ref2 = A.apply(q1), A.apply(q2)
+    dev = torch.device("cuda:0")
+    sa, sb = torch.cuda.Stream(dev), torch.cuda.Stream(dev)
+    q1d, q2d = torch.from_numpy(q1).to(dev), torch.from_numpy(q2).to(dev)
+    p1d = torch.zeros(n, dtype=torch.complex128, device=dev)
+    p2d = torch.zeros(n, dtype=torch.complex128, device=dev)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        A.apply_device(q1d.data_ptr(), p1d.data_ptr(), sa.cuda_stream)
+        A.apply_device(q2d.data_ptr(), p2d.data_ptr(), sb.cuda_stream)
+    torch.cuda.synchronize()
+    A.destroy()
+    assert np.array_equal(p1d.cpu().numpy(), ref1)
+    assert np.array_equal(p2d.cpu().numpy(), ref2)

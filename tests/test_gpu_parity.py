@@ -87,27 +87,29 @@ def test_cube_L6_reference_known_answer(cube, cube_charges, cube_exact):
     np.testing.assert_allclose([linf, l1, l2], [1.38628e-05, 1.22574e-05, 1.13715e-05], rtol=2e-4)
 
 
-@pytest.mark.parametrize("L,kappa,geo", [(3, 0.7, "box"), (5, 40.0, "box"), (7, 25.0, "sphere"),
-                                         (2, 60.0, "sphere"), (8, 12.0, "box")])
-def test_random_clouds_match_oracle(L, kappa, geo):
+@pytest.mark.parametrize("L,kappa,geo,ncrit", [(3, 0.7, "box", 20), (5, 40.0, "box", 10),
+                                               (7, 25.0, "sphere", 20), (2, 60.0, "sphere", 10),
+                                               (8, 12.0, "box", 20), (9, 25.0, "sphere", 20),
+                                               (10, 12.0, "box", 20)])
+def test_random_clouds_match_oracle(L, kappa, geo, ncrit):
+    """Separate source and target clouds; every case has M2L work (the leaf
+    size is chosen so that the tree reaches a precomputation level)."""
     from defmm_amd import workloads as wl
     n = 3000
-    _say(f"case: random {geo} N={n} L={L} kappa={kappa} Ncrit=20")
+    _say(f"case: random {geo} N={n} L={L} kappa={kappa} Ncrit={ncrit}")
     if geo == "box":
         src = np.stack([wl.uniform(100 + k, n) for k in range(3)], 1) * 2 - 1
     else:
         src = wl.sphere(n, seed=5)
     trg = np.stack([wl.uniform(200 + k, n // 2) for k in range(3)], 1) * 1.6 - 0.8
     q = wl.charges(n, seed=3)
-    try:
-        po, so, _, _ = oa.fmm(src, trg, q, L, 20, kappa)
-    except RuntimeError:
-        pytest.skip("oracle: no precomputation level for this case")
-    pg, sg = _run_gpu(src, trg, q, L, 20, kappa)
+    po, so, _, _ = oa.fmm(src, trg, q, L, ncrit, kappa)
+    pg, sg = _run_gpu(src, trg, q, L, ncrit, kappa)
     l2, mx = _diff(pg, po)
     _say(f"  rel L2 {l2:.3e} rel max {mx:.3e} (hf_m2l {sg['hf_m2l']} lf_m2l {sg['lf_m2l']})")
     for k in STAT_KEYS:
         assert sg[k] == so[k], k
+    assert sg["hf_m2l"] + sg["lf_m2l"] > 0
     assert l2 <= FP64_L2 and mx <= FP64_MAX
 
 
@@ -217,7 +219,7 @@ def test_error_codes():
     with pytest.raises(dm.DefmmError, match="no M2L level"):
         A.prcmpt(4, 64, 1.0)   # a single cell: no precomputation level
     with pytest.raises(dm.DefmmError, match="invalid argument"):
-        A.prcmpt(9, 8, 1.0)    # order above 8
+        A.prcmpt(11, 8, 1.0)   # order above 10
 
 
 def test_cpp_dropin_header_reference_scenario(tmp_path):
