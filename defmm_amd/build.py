@@ -2,8 +2,9 @@
 
 The host plan builder (csrc/plan.cpp) is compiled by g++ without
 floating-point contraction (its discrete choices must not depend on the
-compiler); the device executor (csrc/fmm.hip) by hipcc for gfx950 with packed
-FP32 instructions disabled.
+compiler); the device executor (csrc/fmm.hip) and the direct sum
+(csrc/direct.hip) by hipcc for gfx950 with packed FP32 instructions
+disabled.
 """
 import os
 import subprocess
@@ -28,12 +29,14 @@ def build(verbose=True):
     fmm_o = os.path.join(BUILD, "fmm.o")
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-fopenmp",
           "-c", os.path.join(CSRC, "plan.cpp"), "-o", plan_o])
-    _run([HIPCC, "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
-          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-          "-ffp-contract=fast", "-Wno-unused-result",
-          "-c", os.path.join(CSRC, "fmm.hip"), "-o", fmm_o])
+    direct_o = os.path.join(BUILD, "direct.o")
+    for src, obj in (("fmm.hip", fmm_o), ("direct.hip", direct_o)):
+        _run([HIPCC, "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+              "-ffp-contract=fast", "-Wno-unused-result",
+              "-c", os.path.join(CSRC, src), "-o", obj])
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", plan_o, fmm_o, "-o", tmp,
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", plan_o, fmm_o, direct_o, "-o", tmp,
           "-lgomp", "-lpthread"])
     os.replace(tmp, LIB)
     return LIB

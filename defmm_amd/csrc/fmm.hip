@@ -16,13 +16,14 @@
 #include <vector>
 
 #include "../../include/defmm_amd.h"
+#include "errors.hpp"
 #include "grid.hpp"
 #include "kernels.hpp"
 #include "plan.hpp"
 
 namespace dfm {
 
-static thread_local std::string g_err;
+thread_local std::string g_err;  // declared in errors.hpp
 
 // Work-item grids (grid.hpp): lists of millions of items use a 2D grid; a
 // block's item index is blockIdx.y * gridDim.x + blockIdx.x.
@@ -1398,59 +1399,6 @@ __global__ __launch_bounds__(64) void k_p2p_reduce(const int* __restrict__ itemB
   }
 }
 
-// FP64 direct sum at a list of targets (the check of include/interface.hpp:
-// 178-221 on sampled targets): one thread per target, sources staged in LDS,
-// compensated (Kahan) accumulation.
-__global__ __launch_bounds__(256) void k_direct(const double* __restrict__ src, int ns,
-                                                const double2* __restrict__ q,
-                                                const double* __restrict__ trg, int nt,
-                                                double kappa, double2* __restrict__ out) {
-  __shared__ double sx[256], sy[256], sz[256];
-  __shared__ double2 sq[256];
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  const bool act = i < nt;
-  double xt = 0., yt = 0., zt = 0.;
-  if (act) {
-    xt = trg[3 * i];
-    yt = trg[3 * i + 1];
-    zt = trg[3 * i + 2];
-  }
-  double pr = 0., pi = 0., cr = 0., ci = 0.;
-  for (int j0 = 0; j0 < ns; j0 += 256) {
-    const int n = min(256, ns - j0);
-    __syncthreads();
-    if (threadIdx.x < n) {
-      const int j = j0 + threadIdx.x;
-      sx[threadIdx.x] = src[3 * j];
-      sy[threadIdx.x] = src[3 * j + 1];
-      sz[threadIdx.x] = src[3 * j + 2];
-      sq[threadIdx.x] = q[j];
-    }
-    __syncthreads();
-    if (act)
-      for (int j = 0; j < n; j++) {
-        const double x = xt - sx[j], y = yt - sy[j], z = zt - sz[j];
-        const double R = x * x + y * y + z * z;
-        if (R < 1.e-16) continue;
-        const double r = sqrt(R);
-        const double K = 0.079577471545947667884 / r;
-        double si, co;
-        sincos(kappa * r, &si, &co);
-        co *= K;
-        si *= K;
-        const double tr = co * sq[j].x - si * sq[j].y - cr;
-        const double s1 = pr + tr;
-        cr = (s1 - pr) - tr;
-        pr = s1;
-        const double ti = co * sq[j].y + si * sq[j].x - ci;
-        const double s2 = pi + ti;
-        ci = (s2 - pi) - ti;
-        pi = s2;
-      }
-  }
-  if (act) out[i] = make_double2(pr, pi);
-}
-
 // ----------------------------------------------------------------------------
 // executor
 // ----------------------------------------------------------------------------
@@ -2544,42 +2492,6 @@ int defmm_plan_device_bytes(defmm_plan plan, int64_t* bytes) {
   if (!plan || !bytes) { g_err = "null argument"; return DEFMM_EARG; }
   *bytes = plan->x.bytes;
   return DEFMM_OK;
-}
-
-int defmm_direct(const double* src_xyz, int64_t n_src, const double* q, const double* trg_xyz,
-                 int64_t n_trg, double kappa, double* p) {
-  if (!src_xyz || !q || !trg_xyz || !p || n_src < 1 || n_trg < 1 || n_src > 2000000000 ||
-      n_trg > 2000000000) {
-    g_err = "invalid argument";
-    return DEFMM_EARG;
-  }
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) { g_err = "no HIP device"; return DEFMM_ENODEVICE; }
-  void *ds = nullptr, *dq = nullptr, *dt = nullptr, *dp = nullptr;
-  int rc = DEFMM_OK;
-  auto ck = [&](hipError_t e, const char* what) {
-    if (e != hipSuccess && rc == DEFMM_OK) {
-      g_err = std::string(what) + ": " + hipGetErrorString(e);
-      rc = e == hipErrorOutOfMemory ? DEFMM_EMEMORY : DEFMM_EHIP;
-    }
-  };
-  ck(hipMalloc(&ds, n_src * 24), "hipMalloc");
-  if (!rc) ck(hipMalloc(&dq, n_src * 16), "hipMalloc");
-  if (!rc) ck(hipMalloc(&dt, n_trg * 24), "hipMalloc");
-  if (!rc) ck(hipMalloc(&dp, n_trg * 16), "hipMalloc");
-  if (!rc) ck(hipMemcpy(ds, src_xyz, n_src * 24, hipMemcpyHostToDevice), "hipMemcpy");
-  if (!rc) ck(hipMemcpy(dq, q, n_src * 16, hipMemcpyHostToDevice), "hipMemcpy");
-  if (!rc) ck(hipMemcpy(dt, trg_xyz, n_trg * 24, hipMemcpyHostToDevice), "hipMemcpy");
-  if (!rc) {
-    hipLaunchKernelGGL(dfm::k_direct, dim3((unsigned)((n_trg + 255) / 256)), dim3(256), 0, 0,
-                       (const double*)ds, (int)n_src, (const double2*)dq, (const double*)dt,
-                       (int)n_trg, kappa, (double2*)dp);
-    ck(hipGetLastError(), "k_direct launch");
-  }
-  if (!rc) ck(hipMemcpy(p, dp, n_trg * 16, hipMemcpyDeviceToHost), "hipMemcpy");
-  for (void* m : {ds, dq, dt, dp})
-    if (m) (void)hipFree(m);
-  return rc;
 }
 
 int defmm_plan_build_stats(const double* src_xyz, int64_t n_src, const double* trg_xyz,

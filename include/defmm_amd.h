@@ -137,6 +137,24 @@ int defmm_direct(const double* src_xyz, int64_t n_src, const double* q, const do
                  int64_t n_trg, double kappa, double* p);
 
 /*
+ * The same direct sum for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  Device arrays in the layouts
+ * of defmm_direct (src_dev n_src x 3 doubles, q_dev n_src interleaved
+ * complex doubles, trg_dev n_trg x 3, p_dev n_trg complex, overwritten),
+ * enqueued on stream (NULL = default stream) of the current device; returns
+ * after enqueueing.  Results are bit-identical to defmm_direct's.  Calls no
+ * allocating routine: the partial sums live in a fixed 4 MiB array of the
+ * library in each GPU's memory (held until the process ends), so calls that
+ * overlap on one GPU (other streams, other host threads) race and must not
+ * be issued.  n_src, n_trg in 1 .. 2^31-1 (larger sizes are rejected with
+ * DEFMM_EARG); targets are processed in tiles of 4,096 that reuse the array.
+ */
+int defmm_direct_noalloc(const void* src_dev, int64_t n_src, const void* q_dev,
+                         const void* trg_dev, int64_t n_trg, double kappa, void* p_dev,
+                         void* stream);
+
+/*
  * Switch the arithmetic of an existing plan (DEFMM_FP64 / DEFMM_MIXED): the
  * lists stay, the expansion stores and tables are reallocated in the new
  * precision.  Synchronises the plan's work first.
