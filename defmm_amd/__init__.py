@@ -20,7 +20,7 @@ import os
 
 import numpy as np
 
-__all__ = ["IBFMM_Mat", "gemv", "DefmmError", "lib", "FP64", "MIXED", "STAT_NAMES",
+__all__ = ["IBFMM_Mat", "Batch", "gemv", "DefmmError", "lib", "FP64", "MIXED", "STAT_NAMES",
            "STAGE_NAMES", "build_plan_stats", "direct"]
 
 FP64 = 0
@@ -86,6 +86,11 @@ def lib():
     L.defmm_plan_set_m2l_batch_rows.argtypes = [P, ctypes.c_int]
     L.defmm_plan_get_m2l.argtypes = [P, ctypes.POINTER(ctypes.c_int)]
     L.defmm_plan_set_memory_limit.argtypes = [P, ctypes.c_int64]
+    L.defmm_batch_create.argtypes = [ctypes.POINTER(P), ctypes.c_int, ctypes.POINTER(P),
+                                     ctypes.POINTER(P), ctypes.POINTER(ctypes.c_int), ctypes.c_int,
+                                     ctypes.POINTER(P)]
+    L.defmm_batch_execute.argtypes = [P, P]
+    L.defmm_batch_destroy.argtypes = [P]
     L.defmm_direct.argtypes = [D, ctypes.c_int64, D, D, ctypes.c_int64, ctypes.c_double, D]
     _lib = L
     return L
@@ -288,6 +293,39 @@ class IBFMM_Mat:
         if self._plan:
             lib().defmm_plan_destroy(self._plan)
             self._plan = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Independent plans evaluated by one HIP graph (defmm_batch_*): mats is
+    a list of IBFMM_Mat with plans, q_ptrs / p_ptrs device addresses of each
+    instance's charges (nrhs x n_src complex128) and potentials, nrhs the
+    right-hand sides per instance.  The plans must outlive the batch."""
+
+    def __init__(self, mats, q_ptrs, p_ptrs, nrhs, nbranch=0):
+        n = len(mats)
+        for A in mats:
+            A._need_plan()
+        self._mats = list(mats)
+        plans = (ctypes.c_void_p * n)(*[A._plan.value for A in mats])
+        qs = (ctypes.c_void_p * n)(*q_ptrs)
+        ps = (ctypes.c_void_p * n)(*p_ptrs)
+        nr = (ctypes.c_int * n)(*[int(r) for r in nrhs])
+        self._b = ctypes.c_void_p()
+        _check(lib().defmm_batch_create(plans, n, qs, ps, nr, int(nbranch), ctypes.byref(self._b)))
+
+    def execute(self, stream=None):
+        _check(lib().defmm_batch_execute(self._b, ctypes.c_void_p(stream or 0)))
+
+    def destroy(self):
+        if self._b:
+            lib().defmm_batch_destroy(self._b)
+            self._b = ctypes.c_void_p()
 
     def __del__(self):
         try:

@@ -81,10 +81,13 @@ __global__ void k_scatter(const typename C2T<R>::T* __restrict__ p, const int* _
 //   HF leaf, slot of direction d:
 //     M[I] = conj(sig2_d[I]) * sum_j S_I((y_j-c)/r) exp(-i kappa d.(y_j-c)) q_j
 //   LF leaf: M[I] = sum_j S_I((y_j-c)/r) q_j
-// One block per leaf; particles are staged 64 at a time in LDS, and 4 slots
-// (directions) are accumulated per pass over them, sharing the basis products.
+// One block per (leaf, pass of p2mSlotsPerPass(L) slots): particles are
+// staged 64 at a time in LDS, and the pass's slots (directions) are
+// accumulated together, sharing the basis products.  (One block per leaf,
+// looping over its passes, left most of the GPU idle: 1,089 leaves with up
+// to 384 directions at 20,000 points / 32 wavelengths.)
 template <typename R, int L>
-__global__ __launch_bounds__(128) void k_p2m(const int* __restrict__ leaves, DevCells S,
+__global__ __launch_bounds__(128) void k_p2m(const int2* __restrict__ items, DevCells S,
                                              DevLevels lv, const double* __restrict__ px,
                                              const double* __restrict__ py,
                                              const double* __restrict__ pz,
@@ -96,20 +99,20 @@ __global__ __launch_bounds__(128) void k_p2m(const int* __restrict__ leaves, Dev
   typedef typename C2T<R>::T R2;
   constexpr int LL = L * L * L;
   constexpr int NI = (LL + 127) / 128;
-  constexpr int C = L >= 9 ? 2 : 4;  // slots per pass: they share the basis products
+  constexpr int C = p2mSlotsPerPass(L);  // slots per pass: they share the basis products
   __shared__ R Sw[3][L][64];
   __shared__ R2 w[C][64];
   __shared__ double rel[3][64];
   const int item = itemIndex();
   if (item >= nItems) return;
-  const int c = leaves[item];
+  const int2 it = items[item];
+  const int c = it.x, k0 = it.y;
   const int lvl = S.lvl[c], pb = S.pbeg[c], n = S.nprt[c], s0 = S.slot0[c], ns = S.nslot[c];
   const double cx = S.ctr[3 * c], cy = S.ctr[3 * c + 1], cz = S.ctr[3 * c + 2], rad = S.rad[c];
   const int hf = lv.isHF[lvl];
   const int doff = lv.dirOff[lvl], srow = lv.sigRow0[lvl];
-  const bool single = n <= 64;
   const int tid = threadIdx.x;
-  for (int k0 = 0; k0 < ns; k0 += C) {
+  {
     const int kc = min(C, ns - k0);
     R2 acc[NI][C];
 #pragma unroll
@@ -119,7 +122,7 @@ __global__ __launch_bounds__(128) void k_p2m(const int* __restrict__ leaves, Dev
     for (int j0 = 0; j0 < n; j0 += 64) {
       const int cn = min(64, n - j0);
       __syncthreads();
-      if (tid < cn && (!single || k0 == 0)) {
+      if (tid < cn) {
         const int j = tid;
         double r0 = px[pb + j0 + j] - cx, r1 = py[pb + j0 + j] - cy, r2 = pz[pb + j0 + j] - cz;
         rel[0][j] = r0;
@@ -1050,20 +1053,32 @@ __global__ __launch_bounds__(256) void k_zero_slots(const int* __restrict__ slot
 // L2P (L2P.hpp:121-173 with mapOnTrees.hpp:85-134):
 //   HF: p_i += sum_d exp(i kappa d.(x_i-c)) sum_I S_I((x_i-c)/r) sig2_d[I] L_d[I]
 //   LF: p_i += sum_I S_I((x_i-c)/r) L[I]
+// One block of 4 waves per leaf.  The leaf's slots are dealt to NG slot
+// groups: one per wave, or two per wave (half-waves of 32 lanes) when the
+// leaf holds at most 32 points; NG slots at a time are staged in LDS (all of
+// the block's threads load them) and each lane evaluates one point against
+// its group's slot.  The groups' sums are added in group order at the end,
+// so the result does not depend on timing.  (One wave per leaf, looping over
+// its slots one at a time, kept 1,089 waves busy at 20,000 points / 32
+// wavelengths, a third of their lanes idle.)
 template <typename R, int L>
-__global__ __launch_bounds__(64) void k_l2p(const int* __restrict__ leaves, DevCells T,
-                                            DevLevels lv, const double* __restrict__ px,
-                                            const double* __restrict__ py,
-                                            const double* __restrict__ pz,
-                                            const double* __restrict__ dirs,
-                                            const typename C2T<R>::T* __restrict__ sig2,
-                                            const typename C2T<R>::T* __restrict__ Lst,
-                                            OpConsts<R> oc, double kappa,
-                                            typename C2T<R>::T* __restrict__ p, int nItems) {
+constexpr int l2pGroupsMax() {
+  return L * L * L * 2 * (int)sizeof(R) <= 8192 ? 8 : 4;
+}
+template <typename R, int L>
+__global__ __launch_bounds__(256) void k_l2p(const int* __restrict__ leaves, DevCells T,
+                                             DevLevels lv, const double* __restrict__ px,
+                                             const double* __restrict__ py,
+                                             const double* __restrict__ pz,
+                                             const double* __restrict__ dirs,
+                                             const typename C2T<R>::T* __restrict__ sig2,
+                                             const typename C2T<R>::T* __restrict__ Lst,
+                                             OpConsts<R> oc, double kappa,
+                                             typename C2T<R>::T* __restrict__ p, int nItems) {
   typedef typename C2T<R>::T R2;
-  constexpr int LL = L * L * L;
-  constexpr int NI = (LL + 63) / 64;
-  __shared__ R2 buf[LL];
+  constexpr int LL = L * L * L, NWV = 4, NGMAX = l2pGroupsMax<R, L>();
+  __shared__ R2 buf[NGMAX][LL];
+  __shared__ R2 red[NGMAX][64];
   const int item = itemIndex();
   if (item >= nItems) return;
   const int c = leaves[item];
@@ -1071,10 +1086,16 @@ __global__ __launch_bounds__(64) void k_l2p(const int* __restrict__ leaves, DevC
   const double cx = T.ctr[3 * c], cy = T.ctr[3 * c + 1], cz = T.ctr[3 * c + 2], rad = T.rad[c];
   const int hf = lv.isHF[lvl];
   const int doff = lv.dirOff[lvl], srow = lv.sigRow0[lvl];
-  const int tid = threadIdx.x;
-  for (int i0 = 0; i0 < n; i0 += 64) {
-    const int i = i0 + tid;
-    const bool act = i < n;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  // slot groups per wave (uniform over the block) and this lane's group and point
+  const int spw = (n <= 32 && 2 * NWV <= NGMAX) ? 2 : 1;
+  const int NG = NWV * spw < NGMAX ? NWV * spw : NGMAX;
+  const int pts = 64 / spw;
+  const int g = wave * spw + (spw == 2 ? lane >> 5 : 0);
+  const int pl = spw == 2 ? (lane & 31) : lane;
+  for (int i0 = 0; i0 < n; i0 += pts) {
+    const int i = i0 + pl;
+    const bool act = i < n && g < NG;
     double r0 = 0., r1 = 0., r2 = 0.;
     if (act) {
       r0 = px[pb + i] - cx;
@@ -1086,75 +1107,60 @@ __global__ __launch_bounds__(64) void k_l2p(const int* __restrict__ leaves, DevC
     lagrangeAll<R>((R)(r1 / rad), L, oc.xi, oc.iden, sy);
     lagrangeAll<R>((R)(r2 / rad), L, oc.xi, oc.iden, sz);
     R2 pacc = c2<R>(0, 0);
-    // float: the L row (and phase row) of slot k + 1 is loaded into
-    // registers while slot k is evaluated, hiding the load latency (in double
-    // the extra registers cost a wave per SIMD and the kernel was slower:
-    // 21.5 -> 24.1 ms at 1M points / 128 wavelengths; float 10.4 -> 9.5 ms)
-    constexpr bool kAhead = sizeof(R) == 4;
-    R2 nv[NI], np[NI];
-    int nd = 0;
-    auto fetch = [&](int k) {
-      nd = hf ? T.slotDir[s0 + k] : 0;
-#pragma unroll
-      for (int m = 0; m < NI; m++) {
-        const int I = min(tid + 64 * m, LL - 1);
-        nv[m] = Lst[(size_t)(s0 + k) * LL + I];
-        np[m] = hf ? sig2[(size_t)(srow + nd) * LL + I] : c2<R>(1, 0);
-      }
-    };
-    // double from L = 9: the slot's rows go straight to LDS after the
-    // barrier (holding L^3 / 64 pairs of them in registers spilled)
-    constexpr bool kRegs = sizeof(R) == 4 || L <= 8;
-    if (kAhead && ns > 0) fetch(0);
-    for (int k = 0; k < ns; k++) {
-      if (!kAhead && kRegs) fetch(k);
-      int d = nd;
+    for (int k0 = 0; k0 < ns; k0 += NG) {
+      const int kc = min(NG, ns - k0);
       __syncthreads();
-      if (kRegs) {
-#pragma unroll
-        for (int m = 0; m < NI; m++) {
-          const int I = tid + 64 * m;
-          if (I < LL) buf[I] = cmul(nv[m], np[m]);
-        }
-        if (kAhead && k + 1 < ns) fetch(k + 1);
-      } else {
-        d = hf ? T.slotDir[s0 + k] : 0;
-        for (int I = tid; I < LL; I += 64)
-          buf[I] = cmul(Lst[(size_t)(s0 + k) * LL + I],
-                        hf ? sig2[(size_t)(srow + d) * LL + I] : c2<R>(1, 0));
+      // stage the kc slots' L .* sig2 rows
+      for (int x = tid; x < kc * LL; x += 256) {
+        const int u = x / LL, I = x - u * LL, k = s0 + k0 + u;
+        const R2 v = Lst[(size_t)k * LL + I];
+        buf[u][I] = hf ? cmul(v, sig2[(size_t)(srow + T.slotDir[k]) * LL + I]) : v;
       }
       __syncthreads();
-      R2 s = c2<R>(0, 0);
+      if (g < kc) {
+        const R2* b = buf[g];
+        R2 s = c2<R>(0, 0);
 #pragma unroll
-      for (int a = 0; a < L; a++)
+        for (int a = 0; a < L; a++)
 #pragma unroll
-        for (int b = 0; b < L; b++) {
-          R2 s1 = c2<R>(0, 0);
+          for (int bb = 0; bb < L; bb++) {
+            R2 s1 = c2<R>(0, 0);
 #pragma unroll
-          for (int e = 0; e < L; e++) {
-            const R2 v = buf[(a * L + b) * L + e];
-            s1.x += sz[e] * v.x;
-            s1.y += sz[e] * v.y;
+            for (int e = 0; e < L; e++) {
+              const R2 v = b[(a * L + bb) * L + e];
+              s1.x += sz[e] * v.x;
+              s1.y += sz[e] * v.y;
+            }
+            const R t = sx[a] * sy[bb];
+            s.x += t * s1.x;
+            s.y += t * s1.y;
           }
-          const R t = sx[a] * sy[b];
-          s.x += t * s1.x;
-          s.y += t * s1.y;
+        if (hf) {
+          const int d = T.slotDir[s0 + k0 + g];
+          const double dx = dirs[3 * (doff + d)], dy = dirs[3 * (doff + d) + 1],
+                       dz = dirs[3 * (doff + d) + 2];
+          R cs, sn;
+          expi(kappa * (dx * r0 + dy * r1 + dz * r2), cs, sn);
+          s = cmul(s, c2<R>(cs, sn));
         }
-      if (hf) {
-        const double dx = dirs[3 * (doff + d)], dy = dirs[3 * (doff + d) + 1],
-                     dz = dirs[3 * (doff + d) + 2];
-        R cs, sn;
-        expi(kappa * (dx * r0 + dy * r1 + dz * r2), cs, sn);
-        s = cmul(s, c2<R>(cs, sn));
+        pacc.x += s.x;
+        pacc.y += s.y;
       }
-      pacc.x += s.x;
-      pacc.y += s.y;
     }
-    if (act) {
-      R2 o = p[pb + i];
-      o.x += pacc.x;
-      o.y += pacc.y;
-      p[pb + i] = o;
+    // the groups' sums, in group order
+    __syncthreads();
+    if (g < NG) red[g][pl] = pacc;
+    __syncthreads();
+    if (tid < pts && i0 + tid < n) {
+      R2 t = red[0][tid];
+      for (int gg = 1; gg < NG; gg++) {
+        t.x += red[gg][tid].x;
+        t.y += red[gg][tid].y;
+      }
+      R2 o = p[pb + i0 + tid];
+      o.x += t.x;
+      o.y += t.y;
+      p[pb + i0 + tid] = o;
     }
   }
 }
@@ -1458,6 +1464,8 @@ struct Exec {
   void* consts = nullptr;  // xi, iden, A0, A1, B0, B1, twc, tws (real type)
   int *p2m = nullptr, *l2p = nullptr;
   int np2m = 0, nl2p = 0;
+  int2* p2mItems = nullptr;  // (source leaf, first slot of a pass)
+  int np2mItems = 0;
   std::vector<DevTrans> m2m, l2l;
   int *m2fSlot = nullptr;
   int nm2f = 0;
@@ -1510,6 +1518,12 @@ struct Exec {
   hipEvent_t done = nullptr;
   hipStream_t lastStream = nullptr;
   bool doneRecorded = false;
+  // set while the pass is captured into a graph (defmm_batch_create): no
+  // events are recorded or waited on inside the capture
+  bool capturing = false;
+  // incremented by every successful upload of the typed stores: a captured
+  // graph holds their addresses and is stale once it changes
+  uint64_t generation = 0;
 
   void freeTyped() {
     valid = false;
@@ -1814,6 +1828,7 @@ static int uploadPrecision(Exec& X) {
     return rc;
   }
   X.valid = true;
+  X.generation++;
   return OK;
 }
 
@@ -1898,6 +1913,15 @@ static int upload(Exec& X, Plan& pl) {
   }
   CKR(X.upv(X.perm, pl.perm));
   CKR(X.upv(X.p2m, pl.p2mCell)); X.np2m = (int)pl.p2mCell.size();
+  {
+    // P2M work items: every pass of p2mSlotsPerPass(L) slots of a leaf
+    const int C = p2mSlotsPerPass(pl.L);
+    std::vector<int2> it;
+    for (int c : pl.p2mCell)
+      for (int k0 = 0; k0 < std::max(1, pl.S.nslot[c]); k0 += C) it.push_back(make_int2(c, k0));
+    CKR(X.upv(X.p2mItems, it));
+    X.np2mItems = (int)it.size();
+  }
   CKR(X.upv(X.l2p, pl.l2pCell)); X.nl2p = (int)pl.l2pCell.size();
   for (int w = 0; w < 2; w++) {
     auto& src = w ? pl.l2l : pl.m2m;
@@ -2103,7 +2127,9 @@ static int runTyped(Exec& X, const double2* qin, double2* pout, int parts, int n
   const R2* sig = (const R2*)X.sig;
   const R2* sig2 = (const R2*)X.sig2;
   // (a failed record surfaces at the launch check below or in stage_times)
-  auto mark = [&](int k) { (void)hipEventRecord(X.ev[k], s); };
+  auto mark = [&](int k) {
+    if (!X.capturing) (void)hipEventRecord(X.ev[k], s);
+  };
   mark(0);
   hipLaunchKernelGGL(k_zero<R2>, dim3(zeroGrid((int64_t)nrhs * (int64_t)nt, 256)), dim3(256), 0,
                      s, (R2*)X.p, (int64_t)nrhs * (int64_t)nt);
@@ -2113,11 +2139,11 @@ static int runTyped(Exec& X, const double2* qin, double2* pout, int parts, int n
   mark(1);
   const bool far = (parts & 2) != 0, near = (parts & 1) != 0;
 
-  if (far && X.np2m)
+  if (far && X.np2mItems)
     for (int r = 0; r < nrhs; r++)
-      hipLaunchKernelGGL((k_p2m<R, L>), itemGrid(X.np2m), dim3(128), 0, s, X.p2m, X.S, X.lv,
-                         X.sx, X.sy, X.sz, (const R2*)qr(r), X.dirs, sig2, oc, X.kappa, Mr(r),
-                         X.np2m);
+      hipLaunchKernelGGL((k_p2m<R, L>), itemGrid(X.np2mItems), dim3(128), 0, s, X.p2mItems, X.S,
+                         X.lv, X.sx, X.sy, X.sz, (const R2*)qr(r), X.dirs, sig2, oc, X.kappa,
+                         Mr(r), X.np2mItems);
   mark(2);
   if (far)
     for (int r = 0; r < nrhs; r++)
@@ -2172,7 +2198,7 @@ static int runTyped(Exec& X, const double2* qin, double2* pout, int parts, int n
   mark(6);
   if (far && X.nl2p)
     for (int r = 0; r < nrhs; r++)
-      hipLaunchKernelGGL((k_l2p<R, L>), itemGrid(X.nl2p), dim3(64), 0, s, X.l2p, X.T, X.lv,
+      hipLaunchKernelGGL((k_l2p<R, L>), itemGrid(X.nl2p), dim3(256), 0, s, X.l2p, X.T, X.lv,
                          X.tx, X.ty, X.tz, X.dirs, sig2, (const R2*)Lr(r), oc, X.kappa, pr(r),
                          X.nl2p);
   mark(7);
@@ -2193,7 +2219,7 @@ static int runTyped(Exec& X, const double2* qin, double2* pout, int parts, int n
     hipLaunchKernelGGL(k_scatter<R>, dim3(nblk(X.nt, 256)), dim3(256), 0, s, (const R2*)pr(r),
                        X.tperm, pout + r * nt, (int)X.nt);
   mark(9);
-  X.evRecorded = true;
+  if (!X.capturing) X.evRecorded = true;
   HIPCK(hipGetLastError());
   return OK;
 }
@@ -2223,8 +2249,8 @@ static int run(Exec& X, const double2* qin, double2* pout, int parts, int nrhs, 
   }
   HIPCK(hipSetDevice(X.device));
   // every execution uses the plan's stores: one on another stream than the
-  // last starts after it
-  if (X.doneRecorded && s != X.lastStream) HIPCK(hipStreamWaitEvent(s, X.done, 0));
+  // last starts after it (a captured pass is ordered by its batch instead)
+  if (!X.capturing && X.doneRecorded && s != X.lastStream) HIPCK(hipStreamWaitEvent(s, X.done, 0));
   for (int r0 = 0; r0 < nrhs; r0 += X.nr) {
     const int n = std::min(X.nr, nrhs - r0);
     const double2* q = qin + (size_t)r0 * X.ns;
@@ -2233,10 +2259,18 @@ static int run(Exec& X, const double2* qin, double2* pout, int parts, int nrhs, 
                                   : runR<float>(X, q, p, parts, n, s);
     if (rc) return rc;
   }
+  if (X.capturing) return OK;
   HIPCK(hipEventRecord(X.done, s));
   X.lastStream = s;
   X.doneRecorded = true;
   return OK;
+}
+
+// Work of one plan's pass in arbitrary units (near-field pairs plus M2L
+// pair-frequencies, per right-hand side), for balancing graph branches.
+static double passWork(const Exec& X, int nrhs) {
+  return (double)nrhs * ((double)X.st.p2p_pairs + (double)X.st.m2l_entries * X.PP +
+                         (double)(X.st.src_slots + X.st.trg_slots) * X.LL * X.L);
 }
 
 }  // namespace dfm
@@ -2249,6 +2283,26 @@ using dfm::g_err;
 
 struct defmm_plan_s {
   Exec x;
+};
+
+// A batch of independent plans evaluated by one HIP graph (defmm_batch_*).
+struct defmm_batch_s {
+  std::vector<defmm_plan> plans;
+  std::vector<uint64_t> gens;     // plan generations at capture
+  std::vector<int> branchOf;      // branch of each plan
+  std::vector<hipStream_t> branch;
+  std::vector<hipEvent_t> evs;    // fork + one join per branch
+  hipStream_t cs = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int device = 0;
+  ~defmm_batch_s() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (auto e : evs) (void)hipEventDestroy(e);
+    for (auto b : branch) (void)hipStreamDestroy(b);
+    if (cs) (void)hipStreamDestroy(cs);
+  }
 };
 
 extern "C" {
@@ -2513,6 +2567,143 @@ int defmm_plan_build_stats(const double* src_xyz, int64_t n_src, const double* t
   tmp.x.st = pl->st;
   delete pl;
   return defmm_plan_stats(&tmp, out, n);
+}
+
+int defmm_batch_create(const defmm_plan* plans, int n, const void* const* q_dev,
+                       void* const* p_dev, const int* nrhs, int nbranch, defmm_batch* out) {
+  if (!out) { g_err = "null output pointer"; return DEFMM_EARG; }
+  *out = nullptr;
+  if (!plans || !q_dev || !p_dev || !nrhs || n < 1) { g_err = "null argument or empty batch"; return DEFMM_EARG; }
+  if (nbranch < 0) { g_err = "nbranch must be >= 0"; return DEFMM_EARG; }
+  for (int i = 0; i < n; i++) {
+    if (!plans[i] || !q_dev[i] || !p_dev[i] || nrhs[i] < 1) {
+      g_err = "instance " + std::to_string(i) + ": null plan or array, or nrhs < 1";
+      return DEFMM_EARG;
+    }
+    if (!plans[i]->x.valid) {
+      g_err = "instance " + std::to_string(i) + ": the plan is unusable";
+      return DEFMM_ESTATE;
+    }
+    if (plans[i]->x.device != plans[0]->x.device) {
+      g_err = "all plans of a batch must live on one device";
+      return DEFMM_EARG;
+    }
+    for (int j = 0; j < i; j++)
+      if (plans[j] == plans[i]) {
+        g_err = "a plan appears twice in the batch (its stores cannot serve two instances)";
+        return DEFMM_EARG;
+      }
+  }
+  defmm_batch_s* b = new defmm_batch_s();
+  b->device = plans[0]->x.device;
+  auto fail = [&](int code, const std::string& msg) {
+    g_err = msg;
+    delete b;
+    return code;
+  };
+  if (hipSetDevice(b->device) != hipSuccess) return fail(DEFMM_EHIP, "hipSetDevice failed");
+  const int nb = std::max(1, std::min(n, nbranch > 0 ? nbranch : 8));
+  // longest processing time first: instances by decreasing work, each to
+  // the least loaded branch
+  std::vector<int> order(n);
+  for (int i = 0; i < n; i++) order[i] = i;
+  std::vector<double> work(n);
+  for (int i = 0; i < n; i++) work[i] = dfm::passWork(plans[i]->x, nrhs[i]);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return work[a] > work[c]; });
+  std::vector<double> load(nb, 0.);
+  b->branchOf.assign(n, 0);
+  for (int i : order) {
+    const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    b->branchOf[i] = k;
+    load[k] += work[i];
+  }
+  if (hipStreamCreate(&b->cs) != hipSuccess) return fail(DEFMM_EHIP, "hipStreamCreate failed");
+  b->branch.assign(nb, nullptr);
+  for (auto& st : b->branch)
+    if (hipStreamCreate(&st) != hipSuccess) return fail(DEFMM_EHIP, "hipStreamCreate failed");
+  b->evs.assign(nb + 1, nullptr);
+  for (auto& e : b->evs)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+      return fail(DEFMM_EHIP, "hipEventCreate failed");
+  // every plan's last execution has finished before the capture reads its state
+  for (int i = 0; i < n; i++)
+    if (plans[i]->x.quiesce() != dfm::OK) return fail(DEFMM_EHIP, "device synchronisation failed");
+  if (hipStreamBeginCapture(b->cs, hipStreamCaptureModeThreadLocal) != hipSuccess)
+    return fail(DEFMM_EHIP, "hipStreamBeginCapture failed");
+  int rc = DEFMM_OK;
+  std::string why;
+  if (hipEventRecord(b->evs[0], b->cs) != hipSuccess) rc = DEFMM_EHIP;
+  for (int k = 0; k < nb && !rc; k++)
+    if (hipStreamWaitEvent(b->branch[k], b->evs[0], 0) != hipSuccess) rc = DEFMM_EHIP;
+  for (int k = 0; k < nb && !rc; k++)
+    for (int i : order) {
+      if (b->branchOf[i] != k || rc) continue;
+      Exec& X = plans[i]->x;
+      X.capturing = true;
+      rc = dfm::run(X, (const double2*)q_dev[i], (double2*)p_dev[i], 3, nrhs[i], b->branch[k]);
+      X.capturing = false;
+      if (rc) why = "instance " + std::to_string(i) + ": " + g_err;
+    }
+  for (int k = 0; k < nb && !rc; k++)
+    if (hipEventRecord(b->evs[k + 1], b->branch[k]) != hipSuccess ||
+        hipStreamWaitEvent(b->cs, b->evs[k + 1], 0) != hipSuccess)
+      rc = DEFMM_EHIP;
+  hipGraph_t g = nullptr;
+  const hipError_t ec = hipStreamEndCapture(b->cs, &g);
+  b->graph = g;
+  if (rc) return fail(rc, why.empty() ? "capture of the batch failed" : why);
+  if (ec != hipSuccess || !g)
+    return fail(DEFMM_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+  const hipError_t ei = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+  if (ei != hipSuccess)
+    return fail(ei == hipErrorOutOfMemory ? DEFMM_EMEMORY : DEFMM_EHIP,
+                std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+  b->plans.assign(plans, plans + n);
+  for (int i = 0; i < n; i++) b->gens.push_back(plans[i]->x.generation);
+  *out = b;
+  return DEFMM_OK;
+}
+
+int defmm_batch_execute(defmm_batch b, void* stream) {
+  if (!b) { g_err = "null batch"; return DEFMM_EARG; }
+  if (hipSetDevice(b->device) != hipSuccess) { g_err = "hipSetDevice failed"; return DEFMM_EHIP; }
+  hipStream_t s = (hipStream_t)stream;
+  for (size_t i = 0; i < b->plans.size(); i++) {
+    Exec& X = b->plans[i]->x;
+    if (!X.valid || X.generation != b->gens[i]) {
+      g_err = "instance " + std::to_string(i) +
+              ": its plan was reallocated since the batch was created; create the batch again";
+      return DEFMM_ESTATE;
+    }
+    if (X.doneRecorded && X.lastStream != s && hipStreamWaitEvent(s, X.done, 0) != hipSuccess) {
+      g_err = "hipStreamWaitEvent failed";
+      return DEFMM_EHIP;
+    }
+  }
+  const hipError_t e = hipGraphLaunch(b->exec, s);
+  if (e != hipSuccess) {
+    g_err = std::string("hipGraphLaunch: ") + hipGetErrorString(e);
+    return DEFMM_EHIP;
+  }
+  for (defmm_plan p : b->plans) {
+    Exec& X = p->x;
+    if (hipEventRecord(X.done, s) != hipSuccess) {
+      g_err = "hipEventRecord failed";
+      return DEFMM_EHIP;
+    }
+    X.lastStream = s;
+    X.doneRecorded = true;
+    X.evRecorded = false;  // stage times are not recorded inside the graph
+  }
+  return DEFMM_OK;
+}
+
+int defmm_batch_destroy(defmm_batch b) {
+  if (!b) return DEFMM_OK;
+  (void)hipSetDevice(b->device);
+  for (defmm_plan p : b->plans) (void)p->x.quiesce();
+  delete b;
+  return DEFMM_OK;
 }
 
 int defmm_plan_destroy(defmm_plan plan) {

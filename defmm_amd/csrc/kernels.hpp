@@ -72,6 +72,10 @@ struct OpConsts {
   const R* tws;
 };
 
+// Slots of a source leaf accumulated per P2M pass (they share the basis
+// products and the leaf's staged points; fewer from L = 9 for registers).
+constexpr int p2mSlotsPerPass(int L) { return L >= 9 ? 2 : 4; }
+
 // Lagrange weights S_i(t), t in [-1, 1], i < L.
 template <typename R>
 __device__ __forceinline__ void lagrangeAll(R t, int L, const R* xi, const R* iden, R* out) {

@@ -204,6 +204,29 @@ int defmm_plan_set_memory_limit(defmm_plan plan, int64_t bytes);
 /* Bytes of device memory held by the plan. */
 int defmm_plan_device_bytes(defmm_plan plan, int64_t* bytes);
 
+/*
+ * Instance batches.  The reference holds one FMM instance per process
+ * (process-global state, include/global.hpp:45-90); here many independent
+ * plans (sizes, geometries, orders, precisions and right-hand-side counts
+ * may all differ) are evaluated by one HIP graph.  defmm_batch_create
+ * captures every plan's whole pass -- nrhs[i] right-hand sides, charges
+ * q_dev[i] (nrhs[i] x n_src complex128, device) to potentials p_dev[i]
+ * (nrhs[i] x n_trg, device) -- as the graph's branches: the instances are
+ * dealt to nbranch concurrent branches (0 = min(n, 8)), largest work first
+ * to the least loaded branch.  The arrays' addresses are bound into the
+ * graph.  defmm_batch_execute launches the graph on stream (NULL = default
+ * stream) after every plan's previous execution, and returns.  A plan may
+ * appear once per batch; reallocating one of its plans (set_rhs,
+ * set_precision, set_m2l, batch rows) makes the batch stale (DEFMM_ESTATE:
+ * create it again).  Destroy a batch before its plans.  Stage times are not
+ * recorded for batched executions.
+ */
+typedef struct defmm_batch_s* defmm_batch;
+int defmm_batch_create(const defmm_plan* plans, int n, const void* const* q_dev,
+                       void* const* p_dev, const int* nrhs, int nbranch, defmm_batch* out);
+int defmm_batch_execute(defmm_batch batch, void* stream);
+int defmm_batch_destroy(defmm_batch batch);
+
 int defmm_plan_destroy(defmm_plan plan);
 const char* defmm_strerror(int code);
 const char* defmm_last_error(void);
