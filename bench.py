@@ -19,6 +19,12 @@ upload) is timed separately.  Measured on one plan:
 Errors: relative L2 / L-inf on --samples sampled targets against an FP64
 direct sum computed on the GPU.
 
+Instance batch (--batch-instances): the uneven instance mix of the
+benchmark's instance-batch configuration at reduced size (48 instances of
+10,000 - 100,000 points instead of 256 of 10,000 - 500,000, which would not
+fit one GPU's memory at once): each instance evaluated by its own launches
+one after another, then all of them by one HIP graph (defmm_batch_*).
+
 Scale section (--scale-workload, default the 1M-point sphere at 128
 wavelengths, order 5): FP64 and mixed at full size; at this frequency order 5
 gives 7e-3, and the order that reaches 1e-6 does not fit in one GPU's memory
@@ -63,10 +69,13 @@ def log(msg):
 def m2l_nr_per_launch(L, mixed):
     """Right-hand sides one M2L launch serves (the executor's rule: at most 16
     double or 32 float complex accumulators per thread; 1 in double from
-    order 7, 2 in float at order 7)."""
+    order 7, 2 in float at order 7; from order 8 a thread holds a column of
+    2L - 1 frequencies: 1 in double, 2 in float)."""
     PP = (2 * L - 1) ** 3
+    if L >= 8:  # one column of 2L - 1 frequencies per thread
+        return 2 if mixed else 1
     nw = max(-(-PP // 1024), min(-(-PP // 256), 3))
-    g = 2 if L >= 8 else 3 if L == 7 else 4
+    g = 3 if L == 7 else 4
     cap = (32 if mixed else 16) // (g * nw)
     w = 8 if cap >= 8 else 4 if cap >= 4 else 2 if cap >= 2 else 1
     if not mixed and L >= 7:
@@ -83,7 +92,7 @@ def m2l_compulsory_bytes(st, L, nrhs, mixed):
     right-hand side, and the group lists once per launch."""
     PP, LL = (2 * L - 1) ** 3, L ** 3
     s = 8 if mixed else 16
-    g = 2 if L >= 8 else 3 if L == 7 else 4
+    g = 1 if L >= 8 else 3 if L == 7 else 4
     nr = m2l_nr_per_launch(L, mixed)
     launches = -(-nrhs // nr)
     lists = st["m2l_group_entries"] * (4 + 4 + 4 * g) + st["m2l_groups"] * (4 + 4 * g)
@@ -236,6 +245,112 @@ class Runner:
         self.A.destroy()
 
 
+def batch_instances(count, nmin, nmax, seed=5150):
+    """The uneven instance mix of the instance-batch configuration: sizes
+    log-uniform in [nmin, nmax], geometry uniform over {sphere, edge-refined
+    cube, clustered ellipsoid, 16 Gaussian clusters}, 2^U(3,7) wavelengths,
+    order 4-6, 1-16 right-hand sides; one seed per instance."""
+    from defmm_amd import workloads as wl
+    u = wl.uniform(seed, 8 * count)
+    geos = ["sphere", "cube_edges", "ellipsoid", "clusters"]
+    out = []
+    for i in range(count):
+        n = int(10 ** (np.log10(nmin) + (np.log10(nmax) - np.log10(nmin)) * u[8 * i]))
+        geo = geos[min(3, int(u[8 * i + 1] * 4))]
+        nl = 2 ** (3 + 4 * u[8 * i + 2])
+        L = 4 + min(2, int(u[8 * i + 3] * 3))
+        nr = 1 + min(15, int(u[8 * i + 4] * 16))
+        out.append((i, geo, n, nl, L, nr))
+    return out
+
+
+def batch_section(count, nmin, nmax, width, steps, warmup):
+    """Instances evaluated one after another (each plan's own launches) and
+    as one graph (defmm_batch_*); mixed precision; plan width <= `width`
+    right-hand sides (wider instances run in chunks of it)."""
+    import torch
+    import defmm_amd as dm
+    from defmm_amd import workloads as wl
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    mats, Qs, Ps, meta = [], [], [], []
+    t0 = time.time()
+    skipped = 0
+    for (i, geo, n, nl, L, nr) in batch_instances(count, nmin, nmax):
+        pts = wl.GEOMETRIES[geo](n, seed=900 + i)
+        kappa = wl.kappa_for(nl, pts)
+        A = dm.IBFMM_Mat(precision=dm.MIXED)
+        A.setSourceParticles(pts)
+        A.setTargetParticles(pts)
+        try:
+            A.prcmpt(L, 65, kappa)
+            A.set_rhs(min(nr, width))
+        except dm.DefmmError as e:
+            log(f"[batch] instance {i} ({geo} N={n} {nl:.1f} wavelengths L={L}): {e}; left out")
+            A.destroy()
+            skipped += 1
+            continue
+        Q = np.stack([wl.charges(n, seed=1000 + 17 * i + r) for r in range(nr)])
+        mats.append(A)
+        meta.append((i, geo, n, nl, L, nr, pts, kappa, Q[0]))
+        Qs.append(torch.from_numpy(Q).to(dev))
+        Ps.append(torch.zeros((nr, n), dtype=torch.complex128, device=dev))
+    setup_s = time.time() - t0
+    work = sum(m[2] * m[5] for m in meta)
+    dev_gb = sum(A.device_bytes() for A in mats) / 1e9
+    log(f"[batch] {len(mats)} instances ({skipped} left out), {sum(m[2] for m in meta):,} points, "
+        f"{work:,} point-RHS, setup {setup_s:.1f} s, device {dev_gb:.1f} GB")
+
+    def seq():
+        for A, m, q, p in zip(mats, meta, Qs, Ps):
+            A.apply_batch_device(q.data_ptr(), p.data_ptr(), m[5], stream.cuda_stream)
+
+    def timeit(fn):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    ms_seq = timeit(seq)
+    ref = [p.clone() for p in Ps]
+    t0 = time.time()
+    B = dm.Batch(mats, [q.data_ptr() for q in Qs], [p.data_ptr() for p in Ps],
+                 [m[5] for m in meta], nbranch=8)
+    capture_s = time.time() - t0
+    ms_graph = timeit(lambda: B.execute(stream.cuda_stream))
+    same = all(torch.equal(a, b) for a, b in zip(ref, Ps))
+    # accuracy of the first right-hand side on 200 sampled targets of every
+    # fourth instance, against the FP64 direct sum
+    errs = []
+    for k in range(0, len(meta), 4):
+        i, geo, n, nl, L, nr, pts, kappa, q0 = meta[k]
+        idx = np.linspace(0, n - 1, 200).astype(np.int64)
+        ex = dm.direct(pts, q0, pts[idx], kappa)
+        p = Ps[k][0].cpu().numpy()[idx]
+        errs.append((i, L, float(np.sqrt((np.abs(p - ex) ** 2).sum() / (np.abs(ex) ** 2).sum()))))
+    B.destroy()
+    for A in mats:
+        A.destroy()
+    log(f"[batch] sequential {ms_seq:.2f} ms, graph {ms_graph:.2f} ms (capture {capture_s:.2f} s), "
+        f"identical {same}, errors {errs}")
+    return {"workload": f"{len(mats)} instances, {nmin:,}-{nmax:,} points (log-uniform), "
+                        f"sphere / edge-refined cube / clustered ellipsoid / 16 Gaussian clusters, "
+                        f"8-128 wavelengths, order 4-6, 1-16 RHS, mixed precision, leaf<=64",
+            "instances": len(mats), "left_out": skipped, "points": sum(m[2] for m in meta),
+            "point_rhs": work, "setup_s": setup_s, "device_gb": dev_gb,
+            "sequential": {"ms_per_step": ms_seq, "value": work / (ms_seq * 1e-3) / 1e6},
+            "graph": {"ms_per_step": ms_graph, "value": work / (ms_graph * 1e-3) / 1e6,
+                      "capture_s": capture_s, "identical_to_sequential": same},
+            "rel_l2_err_sampled": errs, "unit": "Mpts·RHS/s"}
+
+
 def read_traffic(kernel, workload):
     path = os.path.join(ROOT, "profiles", f"{kernel}_traffic_{workload}.txt")
     if not os.path.exists(path):
@@ -258,6 +373,12 @@ def main():
     ap.add_argument("--scale-workload", default="sphere1m_128l",
                     help="full-size workload timed in FP64 and mixed ('none' to skip)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-instances", type=int, default=48,
+                    help="instances of the uneven instance batch (0 to skip)")
+    ap.add_argument("--batch-sizes", default="10000,100000",
+                    help="smallest,largest instance size of the instance batch")
+    ap.add_argument("--batch-width", type=int, default=2,
+                    help="plan width (right-hand sides held at once) of the batch's instances")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="after timing, copy this many GiB device-to-device (a kernel of known "
                          "traffic, to scale rocprof byte counters)")
@@ -321,6 +442,12 @@ def main():
                  "note": "order 5 at 128 wavelengths: error 7e-3, above the 1e-6 gate"}
         S.close()
 
+    batch = None
+    if args.batch_instances > 0:
+        lo, hi = (int(x) for x in args.batch_sizes.split(","))
+        batch = batch_section(args.batch_instances, lo, hi, args.batch_width,
+                              max(1, min(args.steps, 5)), 1)
+
     cpu = None
     if not args.no_cpu:
         t0 = time.time()
@@ -352,6 +479,7 @@ def main():
         "cpu_baseline": cpu,
         "headline": head,
         "scale": scale,
+        "instance_batch": batch,
     }
     print(json.dumps(out), flush=True)
 

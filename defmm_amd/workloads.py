@@ -123,6 +123,7 @@ CONFIGS = {
     # the FP64 direct sum is <= 1e-6 there (7.2e-7; order 7 gives 5.8e-6)
     "cube20k_32l_L8": ("cube_edges", 20000, 32, 8, 65),
     "cube20k_32l_L9": ("cube_edges", 20000, 32, 9, 65),
+    "cube20k_32l_L7": ("cube_edges", 20000, 32, 7, 65),
     "sphere1m_128l": ("sphere", 1000000, 128, 5, 65),
     # same sphere at 32 wavelengths with order 7: the relative-error <= 1e-6 regime
     "sphere1m_32l_L7": ("sphere", 1000000, 32, 7, 65),
