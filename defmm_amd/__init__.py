@@ -29,6 +29,8 @@ MIXED = 1
 M2L_ON_THE_FLY = 0
 M2L_STORED = 1
 M2L_BATCHED = 2
+# execution options (defmm_amd.h: DEFMM_OPT_*)
+OPT_P2P_MFMA = 1
 
 STAT_NAMES = [
     "p2p_calls", "p2p_pairs", "hf_m2l", "lf_m2l", "src_slots", "trg_slots", "hf_tables",
@@ -86,6 +88,7 @@ def lib():
     L.defmm_plan_set_m2l_batch_rows.argtypes = [P, ctypes.c_int]
     L.defmm_plan_get_m2l.argtypes = [P, ctypes.POINTER(ctypes.c_int)]
     L.defmm_plan_set_memory_limit.argtypes = [P, ctypes.c_int64]
+    L.defmm_plan_set_option.argtypes = [P, ctypes.c_int, ctypes.c_int64]
     L.defmm_batch_create.argtypes = [ctypes.POINTER(P), ctypes.c_int, ctypes.POINTER(P),
                                      ctypes.POINTER(P), ctypes.POINTER(ctypes.c_int), ctypes.c_int,
                                      ctypes.POINTER(P)]
@@ -277,6 +280,12 @@ class IBFMM_Mat:
         ms = np.zeros(len(STAGE_NAMES))
         _check(lib().defmm_stage_times(self._plan, _dptr(ms), len(ms)))
         return dict(zip(STAGE_NAMES, ms.tolist()))
+
+    def set_option(self, option, value):
+        """Execution option (OPT_P2P_MFMA: near field of the mixed path on the
+        matrix cores, 16 right-hand sides at a time; 1 by default)."""
+        self._need_plan()
+        _check(lib().defmm_plan_set_option(self._plan, int(option), int(value)))
 
     def set_memory_limit(self, nbytes):
         """Device memory budget of later reallocations (0: none)."""

@@ -567,13 +567,13 @@ static const int kVariantOfSigma[8] = {0, 1, 0, 2, 3, 0, 4, 5};
 // block of <= 1024 threads covers PP, fewer when 256 threads suffice), TB
 // threads, G target slots per group.  A thread keeps NW F values, G x NW
 // table values and G x NW accumulators in registers.
-// From L = 9 (k_m2l_col) a thread owns one column of the frequency cube: NW
+// From L = 8 (k_m2l_col) a thread owns one column of the frequency cube: NW
 // = P frequencies, TB = P^2 threads rounded up to whole waves, one target.
 template <int L>
 struct M2LShape {
   static constexpr int P = 2 * L - 1;
   static constexpr int PP = P * P * P;
-  static constexpr bool kColumns = L >= 9;
+  static constexpr bool kColumns = L >= 8;
   static constexpr int NWmin = (PP + 1023) / 1024;
   static constexpr int NWpref = (PP + 255) / 256 < 3 ? (PP + 255) / 256 : 3;
   static constexpr int NW = kColumns ? P : NWmin > NWpref ? NWmin : NWpref;
@@ -753,13 +753,16 @@ __attribute__((amdgpu_waves_per_eu(m2lWavesPerEU<R, L, NR>(), 8))) void k_m2l(
   }
 }
 
-// Fused M2L + F2L for orders L >= 9 (same lists and formulas as k_m2l, one
+// Fused M2L + F2L for orders L >= 8 (same lists and formulas as k_m2l, one
 // target per group): thread t < P^2 owns the column (w1, w2) = t of the
 // frequency cube, all P values of w0, so the first axis of the inverse DFT
 // is summed in registers and the Fourier-domain local expansion J never
 // goes to LDS (J, T1 and T2 in LDS would need 198 KB in double at L = 10;
-// T1 and T2 take 88 KB).  Loads of a column are P^2 elements apart and
-// coalesced across the block's threads.
+// T1 and T2 take 88 KB; at L = 8 the grouped kernel's 98 KB allowed one
+// block per CU, whose inverse DFT then overlapped no loads).  Loads of a
+// column are P^2 elements apart and coalesced across the block's threads.
+// Each target sums its entries in the order of the grouped kernel, so both
+// give bit-identical results.
 template <typename R, int L, int NR>
 __global__ __launch_bounds__(M2LShape<L>::TB) void k_m2l_col(
     const int* __restrict__ groupBeg, const int* __restrict__ groupTarget,
@@ -1360,6 +1363,126 @@ __global__ __launch_bounds__(64) void k_p2p(const P2PItem* __restrict__ items,
   }
 }
 
+// P2P for 16 right-hand sides per launch on the matrix cores (float, the
+// mixed-precision path).  An item's near-field sum is the product of the
+// kernel matrix G (targets x sources, complex, evaluated by the VALU one
+// element per lane, double phase reduction as in k_p2p) and the charge matrix
+// Q (sources x 16 right-hand sides):
+//   P_re += G_re Q_re - G_im Q_im,   P_im += G_re Q_im + G_im Q_re,
+// four V_MFMA_F32_16X16X4_F32 per 4 sources.  Operand layouts of that
+// instruction: A (16 x 4) lane l holds row l % 16, column l / 16; B (4 x 16)
+// lane l holds row l / 16, column l % 16; D (16 x 16) lane l holds column
+// l % 16, rows 4 (l / 16) + 0..3.  So lane l evaluates G for target
+// 16 w + l % 16 (wave w of the block's 4) and source 4 k + l / 16 of the
+// current K-step k, and ends holding right-hand side l % 16 of targets
+// 16 w + 4 (l / 16) + 0..3.  Sources are staged 64 at a time in LDS (the
+// charges as [source][right-hand side]); each 64-source tile's sums are added
+// to double accumulators (as k_p2p does) and the item's partial sums go to
+// the layout k_p2p_reduce<float, 16> reads.
+constexpr int kP2PMfmaNR = 16;
+typedef float mfma_v4f __attribute__((ext_vector_type(4)));
+
+template <bool Small>
+__global__ __launch_bounds__(256) void k_p2p_mfma(
+    const P2PItem* __restrict__ items, const int* __restrict__ leafCell,
+    const int* __restrict__ tpbeg, const int* __restrict__ tnprt, const double* __restrict__ tx,
+    const double* __restrict__ ty, const double* __restrict__ tz, const double* __restrict__ sx,
+    const double* __restrict__ sy, const double* __restrict__ sz, const float2* __restrict__ q,
+    size_t qStride, const int* __restrict__ rb, const int* __restrict__ re, double kappa,
+    double2* __restrict__ partial, int nItems) {
+  __shared__ double lx[64], ly[64], lz[64];
+  __shared__ int lj[64];
+  __shared__ float2 lq[64][kP2PMfmaNR];
+  const int item = itemIndex();
+  if (item >= nItems) return;
+  const P2PItem it = items[item];
+  const int c = leafCell[it.leaf];
+  const int tb = tpbeg[c] + it.t0;
+  const int nt = min(64, tnprt[c] - it.t0);
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int ti = wave * 16 + (lane & 15);  // target of this lane's A row
+  const bool tok = ti < nt;
+  const bool waveOn = wave * 16 < nt;      // uniform over the wave
+  double xt = 0., yt = 0., zt = 0.;
+  if (tok) {
+    xt = tx[tb + ti];
+    yt = ty[tb + ti];
+    zt = tz[tb + ti];
+  }
+  const int kq = lane >> 4, rq = lane & 15;
+  mfma_v4f dre = {0.f, 0.f, 0.f, 0.f}, dim = {0.f, 0.f, 0.f, 0.f};
+  double accr[4] = {0., 0., 0., 0.}, acci[4] = {0., 0., 0., 0.};
+  int r = it.r, off = it.off;
+  for (int pos = 0; pos < it.cnt; pos += 64) {
+    const int tileN = min(64, it.cnt - pos);
+    __syncthreads();
+    if (tid < 64) {
+      int j = -1;
+      if (tid < tileN) {
+        int rr = r, oo = off + tid;
+        int len = re[rr] - rb[rr];
+        while (oo >= len) {
+          oo -= len;
+          rr++;
+          len = re[rr] - rb[rr];
+        }
+        j = rb[rr] + oo;
+        lx[tid] = sx[j];
+        ly[tid] = sy[j];
+        lz[tid] = sz[j];
+      }
+      lj[tid] = j;
+    }
+    {
+      int adv = tileN;
+      while (adv > 0) {
+        const int rem = re[r] - rb[r] - off;
+        if (adv < rem) {
+          off += adv;
+          adv = 0;
+        } else {
+          adv -= rem;
+          r++;
+          off = 0;
+        }
+      }
+    }
+    __syncthreads();
+    for (int x = tid; x < 64 * kP2PMfmaNR; x += 256) {
+      const int sj = x / kP2PMfmaNR, rr = x % kP2PMfmaNR;
+      const int j = lj[sj];
+      lq[sj][rr] = j >= 0 ? q[rr * qStride + j] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    if (waveOn) {
+      for (int k = 0; k < 16; k++) {
+        const int j = 4 * k + kq;
+        float gr = 0.f, gi = 0.f;
+        if (tok && j < tileN) p2pKernel<float, Small>(xt, yt, zt, lx[j], ly[j], lz[j], kappa, gr, gi);
+        const float2 b = lq[j][rq];
+        dre = __builtin_amdgcn_mfma_f32_16x16x4f32(gr, b.x, dre, 0, 0, 0);
+        dre = __builtin_amdgcn_mfma_f32_16x16x4f32(-gi, b.y, dre, 0, 0, 0);
+        dim = __builtin_amdgcn_mfma_f32_16x16x4f32(gr, b.y, dim, 0, 0, 0);
+        dim = __builtin_amdgcn_mfma_f32_16x16x4f32(gi, b.x, dim, 0, 0, 0);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        accr[v] += (double)dre[v];
+        acci[v] += (double)dim[v];
+        dre[v] = 0.f;
+        dim[v] = 0.f;
+      }
+    }
+  }
+  if (waveOn)
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int t = wave * 16 + 4 * kq + v;
+      if (t < nt)
+        partial[((size_t)item * kP2PMfmaNR + rq) * 64 + t] = make_double2(accr[v], acci[v]);
+    }
+}
+
 // Sum the partial results of a leaf's items into p (items of a leaf are
 // contiguous, ordered by target chunk).
 // NR right-hand sides (partial layout of k_p2p<R, NR>); right-hand side r
@@ -1489,6 +1612,7 @@ struct Exec {
   size_t mStride = 0, lStride = 0, fStride = 0;  // elements between right-hand sides
   int m2lMode = DEFMM_M2L_STORED;
   int smallArg = 0;          // kappa * (largest point distance) <= 2^20
+  bool p2pMfma = true;       // float P2P in chunks of 16 right-hand sides on the matrix cores
   bool m2lAuto = true;       // mode chosen by free memory (until set explicitly)
   int nr = 1;                // right-hand sides the expansion stores hold
   int batchRows = 16384;     // requested workspace rows of the batched M2L
@@ -2091,6 +2215,23 @@ static void launchP2P(Exec& X, hipStream_t s, int r0) {
                      (size_t)X.nt, X.nleaves);
 }
 
+// 16 right-hand sides r0 .. r0 + 15 of the float path on the matrix cores
+static void launchP2PMfma(Exec& X, hipStream_t s, int r0) {
+  const float2* q = (const float2*)X.q + (size_t)r0 * X.ns;
+  float2* p = (float2*)X.p + (size_t)r0 * X.nt;
+  if (X.smallArg)
+    hipLaunchKernelGGL((k_p2p_mfma<true>), itemGrid(X.nitems), dim3(256), 0, s, X.items,
+                       X.leafCell, X.T.pbeg, X.T.nprt, X.tx, X.ty, X.tz, X.sx, X.sy, X.sz, q,
+                       (size_t)X.ns, X.rb, X.re, X.kappa, X.partial, X.nitems);
+  else
+    hipLaunchKernelGGL((k_p2p_mfma<false>), itemGrid(X.nitems), dim3(256), 0, s, X.items,
+                       X.leafCell, X.T.pbeg, X.T.nprt, X.tx, X.ty, X.tz, X.sx, X.sy, X.sz, q,
+                       (size_t)X.ns, X.rb, X.re, X.kappa, X.partial, X.nitems);
+  hipLaunchKernelGGL((k_p2p_reduce<float, kP2PMfmaNR>), itemGrid(X.nleaves), dim3(64), 0, s,
+                     X.itemBeg, X.items, X.leafCell, X.T.pbeg, X.T.nprt,
+                     (const double2*)X.partial, p, (size_t)X.nt, X.nleaves);
+}
+
 // One evaluation pass for nrhs <= X.nr right-hand sides: qin / pout hold
 // nrhs x n complex128 values, right-hand side after right-hand side, in
 // input order.  Stages as in the reference's gemv; M2L and P2P process up to
@@ -2162,13 +2303,13 @@ static int runTyped(Exec& X, const double2* qin, double2* pout, int parts, int n
       // L (aliasing M) is written for every target slot from here on
       for (int r0 = 0; r0 < nrhs;) r0 += m2lChunk<R, L>(X, s, 0, X.nGroups, r0, nrhs - r0, oc);
     } else if (X.m2lMode == DEFMM_M2L_ON_THE_FLY) {
-      if constexpr (L <= 8) {
+      if constexpr (!M2LShape<L>::kColumns) {
         for (int r = 0; r < nrhs; r++)
           hipLaunchKernelGGL((k_m2l_fly<R, L>), itemGrid(X.nGroups), dim3(M2LShape<L>::TB), 0,
                              s, X.gBeg, X.gTarget, X.gSrc, X.gMask, X.gCode, (const R2*)Mr(r),
                              (const R2*)X.U, Lr(r), oc.twc, oc.tws, X.nGroups);
       } else {
-        g_err = "the on-the-fly M2L variant supports orders up to 8";
+        g_err = "the on-the-fly M2L variant supports orders up to 7";
         return ERR_ARG;
       }
     } else {
@@ -2204,11 +2345,15 @@ static int runTyped(Exec& X, const double2* qin, double2* pout, int parts, int n
   mark(7);
   if (near && X.nitems) {
     // near field: per-item partial sums, then folded into p per target leaf,
-    // up to 8 right-hand sides per kernel value.  (Run on a second stream
-    // beside M2L it measured no faster: both contend for the same CUs.)
+    // up to 8 right-hand sides per kernel value on the VALU, 16 on the
+    // matrix cores in float.  (Run on a second stream beside M2L it measured
+    // no faster: both contend for the same CUs.)
     for (int r0 = 0; r0 < nrhs;) {
       const int n = nrhs - r0;
-      if (n >= 8) { launchP2P<R, 8>(X, s, r0); r0 += 8; }
+      if (sizeof(R) == 4 && n >= kP2PMfmaNR && X.p2pMfma) {
+        launchP2PMfma(X, s, r0);
+        r0 += kP2PMfmaNR;
+      } else if (n >= 8) { launchP2P<R, 8>(X, s, r0); r0 += 8; }
       else if (n >= 4) { launchP2P<R, 4>(X, s, r0); r0 += 4; }
       else if (n >= 2) { launchP2P<R, 2>(X, s, r0); r0 += 2; }
       else { launchP2P<R, 1>(X, s, r0); r0 += 1; }
@@ -2496,8 +2641,8 @@ int defmm_plan_set_m2l(defmm_plan plan, int mode) {
     return DEFMM_EARG;
   }
   Exec& X = plan->x;
-  if (mode == DEFMM_M2L_ON_THE_FLY && X.L > 8) {
-    g_err = "the on-the-fly M2L variant supports orders up to 8";
+  if (mode == DEFMM_M2L_ON_THE_FLY && X.L > 7) {
+    g_err = "the on-the-fly M2L variant supports orders up to 7";
     return DEFMM_EARG;
   }
   if (mode == X.m2lMode && X.valid) {
@@ -2533,6 +2678,20 @@ int defmm_plan_set_m2l_batch_rows(defmm_plan plan, int rows) {
     return DEFMM_EHIP;
   }
   return dfm::reupload(X, X.nr, X.prec, X.m2lMode, X.m2lAuto, oldRows);
+}
+
+int defmm_plan_set_option(defmm_plan plan, int option, int64_t value) {
+  if (!plan) { g_err = "null plan"; return DEFMM_EARG; }
+  Exec& X = plan->x;
+  switch (option) {
+    case DEFMM_OPT_P2P_MFMA:
+      if (value != 0 && value != 1) { g_err = "DEFMM_OPT_P2P_MFMA takes 0 or 1"; return DEFMM_EARG; }
+      if (X.quiesce() != dfm::OK) { g_err = "device synchronisation failed"; return DEFMM_EHIP; }
+      X.p2pMfma = value != 0;
+      return DEFMM_OK;
+  }
+  g_err = "unknown option";
+  return DEFMM_EARG;
 }
 
 int defmm_plan_set_memory_limit(defmm_plan plan, int64_t bytes) {

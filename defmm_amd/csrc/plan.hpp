@@ -21,10 +21,12 @@ typedef std::complex<double> cplx;
 // frequencies, so that the FP64 kernel fits its registers; 3 at L = 7, where
 // the float kernel with two right-hand sides then fits two 768-thread blocks
 // per CU: 1M points / 16 wavelengths, 4 right-hand sides 182.8 -> 159.8 ms,
-// FP64 89.1 -> 91.4 ms; 1 from L = 9, where a thread owns a whole column of
-// 2L - 1 frequencies).
+// FP64 89.1 -> 91.4 ms; 1 from L = 8, where a thread owns a whole column of
+// 2L - 1 frequencies: at 20,000 points / 32 wavelengths / L = 8 the column
+// kernel took M2L from 3.41 to 2.44 ms in float (16 right-hand sides: 38.8 ->
+// 34.8 ms) and from 4.93 to 5.15 ms in double, against groups of 2).
 constexpr int kM2LGroupMax = 8;
-constexpr int m2lGroupSize(int L) { return L >= 9 ? 1 : L == 8 ? 2 : L == 7 ? 3 : 4; }
+constexpr int m2lGroupSize(int L) { return L >= 8 ? 1 : L == 7 ? 3 : 4; }
 
 // Error codes returned through the C ABI.
 enum {

@@ -174,7 +174,7 @@ int defmm_plan_set_precision(defmm_plan plan, int precision);
  *    fills the workspace and one M2L launch reads it while it is still
  *    cache-resident.  Memory: M and L stores plus the workspace.
  *  DEFMM_M2L_ON_THE_FLY: M2L recomputes each F row from M inside the
- *    kernel; no F store.
+ *    kernel; no F store.  Orders up to 7.
  */
 #define DEFMM_M2L_ON_THE_FLY 0
 #define DEFMM_M2L_STORED 1
@@ -200,6 +200,16 @@ int defmm_plan_set_m2l_batch_rows(defmm_plan plan, int rows);
  * the plan stays unusable and executions return DEFMM_ESTATE.
  */
 int defmm_plan_set_memory_limit(defmm_plan plan, int64_t bytes);
+
+/*
+ * Execution options of a plan (no reallocation; waits for the plan's last
+ * execution).  DEFMM_OPT_P2P_MFMA: 1 (default) evaluates the near field of
+ * the mixed-precision path 16 right-hand sides at a time on the matrix cores
+ * when the plan's width allows it; 0 keeps it on the vector units (up to 8
+ * right-hand sides per kernel value).
+ */
+#define DEFMM_OPT_P2P_MFMA 1
+int defmm_plan_set_option(defmm_plan plan, int option, int64_t value);
 
 /* Bytes of device memory held by the plan. */
 int defmm_plan_device_bytes(defmm_plan plan, int64_t* bytes);

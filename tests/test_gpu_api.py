@@ -4,8 +4,8 @@
     times, equal the host-array executions;
   - switching the arithmetic of an existing plan FP64 -> MIXED -> FP64 gives
     the results of fresh plans of each arithmetic;
-  - mixed precision at order 8 (float M2L kernels with 4 right-hand sides per
-    launch) with 5 right-hand sides against FP64;
+  - mixed precision at order 8 (the column M2L kernel, 2 right-hand sides per
+    launch in float) with 5 right-hand sides against FP64;
   - a reallocation that cannot be satisfied (over the plan's memory limit)
     returns an error and leaves the plan usable with its previous settings;
     when even those cannot be restored, executions return an error instead
@@ -107,7 +107,7 @@ def test_mixed_order8_five_rhs():
     pts, kappa = _cloud(5000, seed=13)
     n, nr = len(pts), 5
     Q = np.stack([wl.charges(n, seed=600 + r) for r in range(nr)])
-    _say(f"case: order 8, {nr} right-hand sides (float M2L 4 + 1 per launch) vs FP64")
+    _say(f"case: order 8, {nr} right-hand sides (float M2L 2 + 2 + 1 per launch) vs FP64")
     A = _plan(dm, pts, 8, 65, kappa, dm.FP64)
     st = A.stats()
     assert st["hf_m2l"] + st["lf_m2l"] > 0
@@ -176,3 +176,36 @@ def test_executions_on_two_streams_are_serialised():
     A.destroy()
     assert np.array_equal(p1d.cpu().numpy(), ref1)
     assert np.array_equal(p2d.cpu().numpy(), ref2)
+
+
+def test_p2p_matrix_cores_match_vector_units():
+    """The mixed-precision near field on the matrix cores (16 right-hand
+    sides per kernel value, V_MFMA_F32_16X16X4_F32) against the vector-unit
+    kernel (8 per kernel value) and FP64: 32 right-hand sides (two launches of
+    16) on a leaf-heavy clustered ellipsoid, plan width 16."""
+    import defmm_amd as dm
+    from defmm_amd import workloads as wl
+    pts = wl.ellipsoid_clusters(12000, seed=15)
+    kappa = wl.kappa_for(64, pts)
+    n, nr = len(pts), 32
+    Q = np.stack([wl.charges(n, seed=800 + r) for r in range(nr)])
+    _say(f"case: near field on matrix cores, ellipsoid N={n} 64 wavelengths L=5 Ncrit=97, "
+         f"{nr} right-hand sides")
+    A = _plan(dm, pts, 5, 97, kappa, dm.FP64)
+    st = A.stats()
+    assert st["p2p_pairs"] > 10_000_000
+    A.set_rhs(16)
+    ref = A.apply_batch(Q)
+    A.set_precision(dm.MIXED)
+    A.set_option(dm.OPT_P2P_MFMA, 0)
+    valu = A.apply_batch(Q)
+    A.set_option(dm.OPT_P2P_MFMA, 1)
+    mfma = A.apply_batch(Q)
+    mfma2 = A.apply_batch(Q)
+    A.destroy()
+    assert np.array_equal(mfma, mfma2)
+    d_valu = max(_diff(mfma[r], valu[r])[0] for r in range(nr))
+    d_ref = max(_diff(mfma[r], ref[r])[0] for r in range(nr))
+    _say(f"  matrix cores vs vector units: worst rel L2 {d_valu:.3e}; vs FP64 {d_ref:.3e}")
+    assert d_valu <= 1e-6
+    assert d_ref <= 3e-6
