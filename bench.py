@@ -351,6 +351,34 @@ def batch_section(count, nmin, nmax, width, steps, warmup):
             "rel_l2_err_sampled": errs, "unit": "Mpts·RHS/s"}
 
 
+def multi_rhs_section(name, nrhs, width, steps, warmup, samples=200):
+    """A many-right-hand-side workload at full size: FP64 with one
+    right-hand side, then mixed precision with nrhs right-hand sides evaluated
+    `width` at a time (M2L variant chosen by free memory); errors of the first
+    right-hand side on sampled targets against the FP64 direct sum."""
+    import torch
+    import defmm_amd as dm
+    R = Runner(name, samples, steps, warmup)
+    ms64, e64, _, st64 = R.timed(1, "fp64")
+    R.A.set_precision(dm.MIXED)
+    R.A.set_rhs(width)
+    msb, eb, _, stb = R.timed(nrhs, f"mixed x{nrhs} (width {width})")
+    out = {"workload": R.describe(), "setup_s": R.setup_s, "setup_ms_by_phase": R.setup_ms,
+           "plan": {k: R.st[k] for k in ("p2p_pairs", "hf_m2l", "src_slots", "trg_slots",
+                                         "m2f_slots", "m2l_table_rows")},
+           "fp64": {"value": R.n / (ms64 * 1e-3) / 1e6, "ms_per_step": ms64, "rel_l2_err": e64,
+                    "stage_ms": st64},
+           "mixed_batched": {"value": R.n * nrhs / (msb * 1e-3) / 1e6, "nrhs": nrhs,
+                             "width": width, "ms_per_step": msb, "rel_l2_err_first_rhs": eb,
+                             "stage_ms": stb, "speedup_vs_fp64": ms64 * nrhs / msb,
+                             "m2l_variant": ["on the fly", "stored", "batched"][R.A.m2l_mode()],
+                             "device_gb": R.A.device_bytes() / 1e9},
+           "unit": "Mpts·RHS/s"}
+    R.close()
+    torch.cuda.empty_cache()
+    return out
+
+
 def read_traffic(kernel, workload):
     path = os.path.join(ROOT, "profiles", f"{kernel}_traffic_{workload}.txt")
     if not os.path.exists(path):
@@ -377,6 +405,11 @@ def main():
                     help="instances of the uneven instance batch (0 to skip)")
     ap.add_argument("--batch-sizes", default="10000,100000",
                     help="smallest,largest instance size of the instance batch")
+    ap.add_argument("--multi-rhs-workload", default="none",
+                    help="full-size many-right-hand-side workload, e.g. ellipsoid2m_256l_L6 "
+                         "('none' to skip)")
+    ap.add_argument("--multi-rhs", type=int, default=64)
+    ap.add_argument("--multi-rhs-width", type=int, default=16)
     ap.add_argument("--batch-width", type=int, default=2,
                     help="plan width (right-hand sides held at once) of the batch's instances")
     ap.add_argument("--copy-gib", type=float, default=0.0,
@@ -442,6 +475,11 @@ def main():
                  "note": "order 5 at 128 wavelengths: error 7e-3, above the 1e-6 gate"}
         S.close()
 
+    many = None
+    if args.multi_rhs_workload != "none":
+        many = multi_rhs_section(args.multi_rhs_workload, args.multi_rhs, args.multi_rhs_width,
+                                 max(1, min(args.steps, 3)), 1)
+
     batch = None
     if args.batch_instances > 0:
         lo, hi = (int(x) for x in args.batch_sizes.split(","))
@@ -480,6 +518,7 @@ def main():
         "headline": head,
         "scale": scale,
         "instance_batch": batch,
+        "many_rhs": many,
     }
     print(json.dumps(out), flush=True)
 
