@@ -1168,6 +1168,158 @@ __global__ __launch_bounds__(256) void k_l2p(const int* __restrict__ leaves, Dev
   }
 }
 
+// L2P for 16 right-hand sides on the matrix cores (float path).  For a
+// target leaf and each of its slots s, C_s = S B_s with S (points x L^3,
+// real) the Lagrange weights S_I((x_i - c)/r) and B_s (L^3 x 16, complex) the
+// slot's local expansions times its phase row, B_s[I][r] = sig2_s[I]
+// L_s,r[I]; then p_r[i] += exp(i kappa d_s.(x_i - c)) C_s[i][r] (the sums of
+// L2P.hpp:121-173).  One block of 4 waves per (leaf, tile of up to TP
+// points): the tile's S is staged in LDS once for all slots; wave w takes
+// the slots k = w mod 4, each as ceil(L^3 / 4) steps of V_MFMA_F32_16X16X4_F32
+// (real and imaginary part of B, TP / 16 row blocks sharing each B operand)
+// with A = S (lane l: point l % 16 of a row block, weight index 4 k + l / 16)
+// and B (lane l: weight index 4 k + l / 16, right-hand side l % 16).  The
+// waves' sums are added in wave order at the end (deterministic).
+constexpr int kL2PMfmaNR = 16;
+template <int L>
+constexpr int l2pMfmaTile() {
+  return L * L * L <= 512 ? 32 : 16;
+}
+template <int L>
+__global__ __launch_bounds__(256) void k_l2p_mfma(
+    const int2* __restrict__ items, DevCells T, DevLevels lv, const double* __restrict__ px,
+    const double* __restrict__ py, const double* __restrict__ pz,
+    const double* __restrict__ dirs, const float2* __restrict__ sig2,
+    const float2* __restrict__ Lst, size_t lStride, OpConsts<float> oc, double kappa,
+    float2* __restrict__ p, size_t pStride, int nItems) {
+  constexpr int LL = L * L * L, KS = (LL + 3) / 4, TP = l2pMfmaTile<L>(), NB = TP / 16;
+  // S (TP x 4 KS floats) and, once every wave is past the slot loop, the
+  // waves' sums (4 x TP x 16 complex) share one array
+  constexpr int NS = TP * KS * 4, NRED = 4 * TP * kL2PMfmaNR * 2;
+  __shared__ float smem[NS > NRED ? NS : NRED];
+  float(*S)[KS * 4] = reinterpret_cast<float(*)[KS * 4]>(smem);
+  float2(*red)[TP][kL2PMfmaNR] = reinterpret_cast<float2(*)[TP][kL2PMfmaNR]>(smem);
+  __shared__ float W[3][TP][L];
+  __shared__ double rel[3][TP];
+  const int item = itemIndex();
+  if (item >= nItems) return;
+  const int2 it = items[item];
+  const int c = it.x, i0 = it.y;
+  const int lvl = T.lvl[c], pb = T.pbeg[c], n = T.nprt[c], s0 = T.slot0[c], ns = T.nslot[c];
+  const double cx = T.ctr[3 * c], cy = T.ctr[3 * c + 1], cz = T.ctr[3 * c + 2], rad = T.rad[c];
+  const int hf = lv.isHF[lvl];
+  const int doff = lv.dirOff[lvl], srow = lv.sigRow0[lvl];
+  const int nt = min(TP, n - i0);
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  // Lagrange weights of the tile's points (zero rows beyond the leaf)
+  if (tid < TP) {
+    const int i = tid;
+    double r0 = 0., r1 = 0., r2 = 0.;
+    if (i < nt) {
+      r0 = px[pb + i0 + i] - cx;
+      r1 = py[pb + i0 + i] - cy;
+      r2 = pz[pb + i0 + i] - cz;
+    }
+    rel[0][i] = r0;
+    rel[1][i] = r1;
+    rel[2][i] = r2;
+    float t[LMAX];
+    const float on = i < nt ? 1.f : 0.f;
+    lagrangeAll<float>((float)(r0 / rad), L, oc.xi, oc.iden, t);
+#pragma unroll
+    for (int a = 0; a < L; a++) W[0][i][a] = t[a] * on;
+    lagrangeAll<float>((float)(r1 / rad), L, oc.xi, oc.iden, t);
+#pragma unroll
+    for (int a = 0; a < L; a++) W[1][i][a] = t[a];
+    lagrangeAll<float>((float)(r2 / rad), L, oc.xi, oc.iden, t);
+#pragma unroll
+    for (int a = 0; a < L; a++) W[2][i][a] = t[a];
+  }
+  __syncthreads();
+  for (int x = tid; x < TP * KS * 4; x += 256) {
+    const int i = x / (KS * 4), I = x - i * (KS * 4);
+    float v = 0.f;
+    if (I < LL) v = W[0][i][I / (L * L)] * W[1][i][(I / L) % L] * W[2][i][I % L];
+    S[i][I] = v;
+  }
+  __syncthreads();
+  const int lr = lane & 15, j = lane >> 4;
+  float2 pacc[NB][4];
+#pragma unroll
+  for (int b = 0; b < NB; b++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) pacc[b][v] = make_float2(0.f, 0.f);
+  for (int k = wave; k < ns; k += 4) {
+    const int sl = s0 + k;
+    const int d = hf ? T.slotDir[sl] : 0;
+    const float2* sg = sig2 + (size_t)(srow + d) * LL;
+    const float2* Ls = Lst + (size_t)lr * lStride + (size_t)sl * LL;  // right-hand side lr
+    mfma_v4f cre[NB], cim[NB];
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+      cre[b] = mfma_v4f{0.f, 0.f, 0.f, 0.f};
+      cim[b] = mfma_v4f{0.f, 0.f, 0.f, 0.f};
+    }
+    for (int ks = 0; ks < KS; ks++) {
+      const int kk = 4 * ks + j;
+      float2 bv = make_float2(0.f, 0.f);
+      if (kk < LL) {
+        const float2 lv2 = Ls[kk];
+        bv = hf ? cmul(lv2, sg[kk]) : lv2;
+      }
+#pragma unroll
+      for (int b = 0; b < NB; b++) {
+        const float a = S[16 * b + lr][kk];
+        cre[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bv.x, cre[b], 0, 0, 0);
+        cim[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bv.y, cim[b], 0, 0, 0);
+      }
+    }
+    // this lane holds C[point 16 b + 4 j + v][right-hand side lr]
+    double dx = 0., dy = 0., dz = 0.;
+    if (hf) {
+      dx = dirs[3 * (doff + d)];
+      dy = dirs[3 * (doff + d) + 1];
+      dz = dirs[3 * (doff + d) + 2];
+    }
+#pragma unroll
+    for (int b = 0; b < NB; b++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int i = 16 * b + 4 * j + v;
+        float2 cv = make_float2(cre[b][v], cim[b][v]);
+        if (hf) {
+          float cs, sn;
+          expi(kappa * (dx * rel[0][i] + dy * rel[1][i] + dz * rel[2][i]), cs, sn);
+          cv = cmul(cv, make_float2(cs, sn));
+        }
+        pacc[b][v].x += cv.x;
+        pacc[b][v].y += cv.y;
+      }
+  }
+  __syncthreads();  // every wave is done with S
+#pragma unroll
+  for (int b = 0; b < NB; b++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) red[wave][16 * b + 4 * j + v][lr] = pacc[b][v];
+  __syncthreads();
+  for (int x = tid; x < TP * kL2PMfmaNR; x += 256) {
+    const int i = x / kL2PMfmaNR, r = x % kL2PMfmaNR;
+    if (i < nt) {
+      float2 t = red[0][i][r];
+#pragma unroll
+      for (int w = 1; w < 4; w++) {
+        t.x += red[w][i][r].x;
+        t.y += red[w][i][r].y;
+      }
+      float2* o = p + (size_t)r * pStride + pb + i0 + i;
+      float2 ov = *o;
+      ov.x += t.x;
+      ov.y += t.y;
+      *o = ov;
+    }
+  }
+}
+
 // P2P near field (P2P.hpp:42-79): p_i += sum_j exp(i kappa r)/(4 pi r) q_j,
 // r = |x_i - y_j|, term 0 when r^2 < 1e-16.  A work item is up to 64 targets
 // of one leaf against a run of its source list; with nt targets, the wave's
@@ -1380,7 +1532,6 @@ __global__ __launch_bounds__(64) void k_p2p(const P2PItem* __restrict__ items,
 // to double accumulators (as k_p2p does) and the item's partial sums go to
 // the layout k_p2p_reduce<float, 16> reads.
 constexpr int kP2PMfmaNR = 16;
-typedef float mfma_v4f __attribute__((ext_vector_type(4)));
 
 template <bool Small>
 __global__ __launch_bounds__(256) void k_p2p_mfma(
@@ -1589,6 +1740,11 @@ struct Exec {
   int np2m = 0, nl2p = 0;
   int2* p2mItems = nullptr;  // (source leaf, first slot of a pass)
   int np2mItems = 0;
+  // (target leaf, first point) of the matrix-core L2P's point tiles, per
+  // tile size (16 or 32 points: the order's l2pMfmaTile)
+  int2* l2pTiles = nullptr;
+  int nl2pTiles = 0;
+  bool l2pMfma = true;       // float L2P in chunks of 16 right-hand sides on the matrix cores
   std::vector<DevTrans> m2m, l2l;
   int *m2fSlot = nullptr;
   int nm2f = 0;
@@ -2047,6 +2203,15 @@ static int upload(Exec& X, Plan& pl) {
     X.np2mItems = (int)it.size();
   }
   CKR(X.upv(X.l2p, pl.l2pCell)); X.nl2p = (int)pl.l2pCell.size();
+  {
+    // tiles of the matrix-core L2P: 32 points up to L^3 = 512, else 16
+    const int tp = pl.LL <= 512 ? 32 : 16;
+    std::vector<int2> it;
+    for (int c : pl.l2pCell)
+      for (int i0 = 0; i0 < std::max(1, pl.T.nprt[c]); i0 += tp) it.push_back(make_int2(c, i0));
+    CKR(X.upv(X.l2pTiles, it));
+    X.nl2pTiles = (int)it.size();
+  }
   for (int w = 0; w < 2; w++) {
     auto& src = w ? pl.l2l : pl.m2m;
     auto& dst = w ? X.l2l : X.m2m;
@@ -2338,10 +2503,22 @@ static int runTyped(Exec& X, const double2* qin, double2* pout, int parts, int n
         launchTrans<R, L>(t, (const R2*)Lr(r), Lr(r), sig2, sig, oc.B[0], oc.B[1], 1, s);
   mark(6);
   if (far && X.nl2p)
-    for (int r = 0; r < nrhs; r++)
+    for (int r = 0; r < nrhs;) {
+      if constexpr (sizeof(R) == 4) {
+        if (nrhs - r >= kL2PMfmaNR && X.l2pMfma) {
+          hipLaunchKernelGGL((k_l2p_mfma<L>), itemGrid(X.nl2pTiles), dim3(256), 0, s,
+                             X.l2pTiles, X.T, X.lv, X.tx, X.ty, X.tz, X.dirs, (const float2*)sig2,
+                             (const float2*)Lr(r), X.lStride, oc, X.kappa, (float2*)pr(r),
+                             (size_t)nt, X.nl2pTiles);
+          r += kL2PMfmaNR;
+          continue;
+        }
+      }
       hipLaunchKernelGGL((k_l2p<R, L>), itemGrid(X.nl2p), dim3(256), 0, s, X.l2p, X.T, X.lv,
                          X.tx, X.ty, X.tz, X.dirs, sig2, (const R2*)Lr(r), oc, X.kappa, pr(r),
                          X.nl2p);
+      r++;
+    }
   mark(7);
   if (near && X.nitems) {
     // near field: per-item partial sums, then folded into p per target leaf,
@@ -2685,9 +2862,10 @@ int defmm_plan_set_option(defmm_plan plan, int option, int64_t value) {
   Exec& X = plan->x;
   switch (option) {
     case DEFMM_OPT_P2P_MFMA:
-      if (value != 0 && value != 1) { g_err = "DEFMM_OPT_P2P_MFMA takes 0 or 1"; return DEFMM_EARG; }
+    case DEFMM_OPT_L2P_MFMA:
+      if (value != 0 && value != 1) { g_err = "the option takes 0 or 1"; return DEFMM_EARG; }
       if (X.quiesce() != dfm::OK) { g_err = "device synchronisation failed"; return DEFMM_EHIP; }
-      X.p2pMfma = value != 0;
+      (option == DEFMM_OPT_P2P_MFMA ? X.p2pMfma : X.l2pMfma) = value != 0;
       return DEFMM_OK;
   }
   g_err = "unknown option";

@@ -72,6 +72,9 @@ struct OpConsts {
   const R* tws;
 };
 
+// Accumulator of V_MFMA_F32_16X16X4_F32 (4 floats per lane).
+typedef float mfma_v4f __attribute__((ext_vector_type(4)));
+
 // Slots of a source leaf accumulated per P2M pass (they share the basis
 // products and the leaf's staged points; fewer from L = 9 for registers).
 constexpr int p2mSlotsPerPass(int L) { return L >= 9 ? 2 : 4; }

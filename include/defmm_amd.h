@@ -206,9 +206,12 @@ int defmm_plan_set_memory_limit(defmm_plan plan, int64_t bytes);
  * execution).  DEFMM_OPT_P2P_MFMA: 1 (default) evaluates the near field of
  * the mixed-precision path 16 right-hand sides at a time on the matrix cores
  * when the plan's width allows it; 0 keeps it on the vector units (up to 8
- * right-hand sides per kernel value).
+ * right-hand sides per kernel value).  DEFMM_OPT_L2P_MFMA: the same for L2P
+ * (1 by default: 16 right-hand sides per pass over a leaf's slots on the
+ * matrix cores; 0: one right-hand side per pass on the vector units).
  */
 #define DEFMM_OPT_P2P_MFMA 1
+#define DEFMM_OPT_L2P_MFMA 2
 int defmm_plan_set_option(defmm_plan plan, int option, int64_t value);
 
 /* Bytes of device memory held by the plan. */

@@ -209,3 +209,35 @@ def test_p2p_matrix_cores_match_vector_units():
     _say(f"  matrix cores vs vector units: worst rel L2 {d_valu:.3e}; vs FP64 {d_ref:.3e}")
     assert d_valu <= 1e-6
     assert d_ref <= 3e-6
+
+
+@pytest.mark.parametrize("L", [4, 5, 8, 9])
+def test_l2p_matrix_cores_match_vector_units(L):
+    """L2P of the mixed path on the matrix cores (16 right-hand sides per pass
+    over a leaf's slots, tiles of 32 points up to L^3 = 512, else 16) against
+    the vector-unit kernel, and the whole pass against FP64: 16 right-hand
+    sides on the edge-refined cube (leaves of up to 64 points: several point
+    tiles per leaf), plan width 16."""
+    import defmm_amd as dm
+    from defmm_amd import workloads as wl
+    pts = wl.cube_edges(8000, seed=16)
+    kappa = wl.kappa_for(16, pts)
+    n, nr = len(pts), 16
+    Q = np.stack([wl.charges(n, seed=900 + r) for r in range(nr)])
+    _say(f"case: L2P on matrix cores, edge-refined cube N={n} 16 wavelengths L={L} Ncrit=65, "
+         f"{nr} right-hand sides")
+    A = _plan(dm, pts, L, 65, kappa, dm.FP64)
+    assert A.stats()["hf_l2p"] + A.stats()["lf_l2p"] > 0
+    A.set_rhs(nr)
+    ref = A.apply_batch(Q)
+    A.set_precision(dm.MIXED)
+    A.set_option(dm.OPT_L2P_MFMA, 0)
+    valu = A.apply_batch(Q)
+    A.set_option(dm.OPT_L2P_MFMA, 1)
+    mfma = A.apply_batch(Q)
+    A.destroy()
+    d_valu = max(_diff(mfma[r], valu[r])[0] for r in range(nr))
+    d_ref = max(_diff(mfma[r], ref[r])[0] for r in range(nr))
+    _say(f"  matrix cores vs vector units: worst rel L2 {d_valu:.3e}; vs FP64 {d_ref:.3e}")
+    assert d_valu <= 1e-6
+    assert d_ref <= 3e-6
