@@ -32,6 +32,7 @@ M2L_BATCHED = 2
 # execution options (defmm_amd.h: DEFMM_OPT_*)
 OPT_P2P_MFMA = 1
 OPT_L2P_MFMA = 2
+OPT_P2M_MFMA = 3
 
 STAT_NAMES = [
     "p2p_calls", "p2p_pairs", "hf_m2l", "lf_m2l", "src_slots", "trg_slots", "hf_tables",

@@ -195,6 +195,124 @@ __global__ __launch_bounds__(128) void k_p2m(const int2* __restrict__ items, Dev
   }
 }
 
+// P2M for 16 right-hand sides on the matrix cores (float path), leaves of at
+// most 64 points:
+//   M_s,r = conj(sig2_s) .* (S W_s,r),  W_s,r[j] = q_r[j] exp(-i kappa d_s.(y_j - c))
+// with S (L^3 x points) the Lagrange weights (P2M.hpp:113-136; LF leaves
+// without the phases).  One block of 4 waves per (leaf, range of up to 32
+// slots); the leaf's points, 1D weights and charges are staged in LDS, and
+// wave w takes the range's slots w mod 4.  For a slot the wave writes W (64
+// points x 16 right-hand sides) to LDS, then for every 16-row block of L^3
+// runs ceil(n / 4) steps of V_MFMA_F32_16X16X4_F32 for the real and the
+// imaginary part (A: lane l row I = 16 m + l % 16, point 4 k + l / 16, the
+// product of the point's three 1D weights; B: point 4 k + l / 16,
+// right-hand side l % 16) and stores the 16 x 16 result times conj(sig2_s).
+constexpr int kP2MMfmaNR = 16, kP2MMfmaSlots = 32, kP2MMfmaPoints = 64;
+template <int L>
+__global__ __launch_bounds__(256) void k_p2m_mfma(
+    const int2* __restrict__ items, DevCells S, DevLevels lv, const double* __restrict__ px,
+    const double* __restrict__ py, const double* __restrict__ pz,
+    const float2* __restrict__ q, size_t qStride, const double* __restrict__ dirs,
+    const float2* __restrict__ sig2, OpConsts<float> oc, double kappa,
+    float2* __restrict__ M, size_t mStride, int nItems) {
+  constexpr int LL = L * L * L, MT = (LL + 15) / 16, NP = kP2MMfmaPoints;
+  __shared__ float Wt[3][NP][L];
+  __shared__ double rel[3][NP];
+  __shared__ float2 qs[NP][kP2MMfmaNR];
+  __shared__ float2 Bw[4][NP][kP2MMfmaNR];
+  const int item = itemIndex();
+  if (item >= nItems) return;
+  const int2 it = items[item];
+  const int c = it.x, k0 = it.y;
+  const int lvl = S.lvl[c], pb = S.pbeg[c], n = min(S.nprt[c], NP), s0 = S.slot0[c],
+            ns = S.nslot[c];
+  const double cx = S.ctr[3 * c], cy = S.ctr[3 * c + 1], cz = S.ctr[3 * c + 2], rad = S.rad[c];
+  const int hf = lv.isHF[lvl];
+  const int doff = lv.dirOff[lvl], srow = lv.sigRow0[lvl];
+  const int kc = min(kP2MMfmaSlots, ns - k0);
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  if (tid < NP) {
+    const int i = tid;
+    double r0 = 0., r1 = 0., r2 = 0.;
+    if (i < n) {
+      r0 = px[pb + i] - cx;
+      r1 = py[pb + i] - cy;
+      r2 = pz[pb + i] - cz;
+    }
+    rel[0][i] = r0;
+    rel[1][i] = r1;
+    rel[2][i] = r2;
+    float t[LMAX];
+    const float on = i < n ? 1.f : 0.f;
+    lagrangeAll<float>((float)(r0 / rad), L, oc.xi, oc.iden, t);
+#pragma unroll
+    for (int a = 0; a < L; a++) Wt[0][i][a] = t[a] * on;
+    lagrangeAll<float>((float)(r1 / rad), L, oc.xi, oc.iden, t);
+#pragma unroll
+    for (int a = 0; a < L; a++) Wt[1][i][a] = t[a];
+    lagrangeAll<float>((float)(r2 / rad), L, oc.xi, oc.iden, t);
+#pragma unroll
+    for (int a = 0; a < L; a++) Wt[2][i][a] = t[a];
+  }
+  for (int x = tid; x < NP * kP2MMfmaNR; x += 256) {
+    const int i = x / kP2MMfmaNR, r = x % kP2MMfmaNR;
+    qs[i][r] = i < n ? q[(size_t)r * qStride + pb + i] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  const int lr = lane & 15, j = lane >> 4;
+  const int ksn = (n + 3) / 4;
+  for (int u = wave; u < kc; u += 4) {
+    const int sl = s0 + k0 + u;
+    const int d = hf ? S.slotDir[sl] : 0;
+    // W of this slot: lane = point
+    {
+      float2 ph = make_float2(1.f, 0.f);
+      if (hf) {
+        const double dx = dirs[3 * (doff + d)], dy = dirs[3 * (doff + d) + 1],
+                     dz = dirs[3 * (doff + d) + 2];
+        float cs, sn;
+        expi(-kappa * (dx * rel[0][lane] + dy * rel[1][lane] + dz * rel[2][lane]), cs, sn);
+        ph = make_float2(cs, sn);
+      }
+#pragma unroll
+      for (int r = 0; r < kP2MMfmaNR; r++) Bw[wave][lane][r] = cmul(qs[lane][r], ph);
+    }
+    // the wave's LDS writes are visible to its own lanes
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const float2* sgr = sig2 + (size_t)(srow + d) * LL;
+    for (int mt = 0; mt < MT; mt++) {
+      const int I = 16 * mt + lr;
+      const bool rowOn = I < LL;
+      const int a0 = rowOn ? I / (L * L) : 0, a1 = rowOn ? (I / L) % L : 0,
+                a2 = rowOn ? I % L : 0;
+      mfma_v4f dre = {0.f, 0.f, 0.f, 0.f}, dim = {0.f, 0.f, 0.f, 0.f};
+      for (int ks = 0; ks < ksn; ks++) {
+        const int k = 4 * ks + j;
+        const float a = rowOn ? Wt[0][k][a0] * Wt[1][k][a1] * Wt[2][k][a2] : 0.f;
+        const float2 b = Bw[wave][k][lr];
+        dre = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b.x, dre, 0, 0, 0);
+        dim = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b.y, dim, 0, 0, 0);
+      }
+      // rows 16 mt + 4 j + v, right-hand side lr
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int Io = 16 * mt + 4 * j + v;
+        if (Io < LL) {
+          float2 val = make_float2(dre[v], dim[v]);
+          if (hf) val = cmulc(val, sgr[Io]);
+          M[(size_t)lr * mStride + (size_t)sl * LL + Io] = val;
+        }
+      }
+    }
+    // this slot's W is read before the next slot's is written
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
 // Separable translation of one level (M2M: M2M.hpp:34-124 with the sweeps of
 // interpolation.hpp:244-361; L2L: L2L.hpp:34-127 with interpolation.hpp:
 // 395-513).  For every output slot o:
@@ -1740,6 +1858,11 @@ struct Exec {
   int np2m = 0, nl2p = 0;
   int2* p2mItems = nullptr;  // (source leaf, first slot of a pass)
   int np2mItems = 0;
+  // matrix-core P2M: (leaf, first slot of a range of 32) for leaves of <= 64
+  // points, and the vector-unit items of the larger leaves
+  int2 *p2mMfmaItems = nullptr, *p2mBigItems = nullptr;
+  int np2mMfma = 0, np2mBig = 0;
+  bool p2mMfma = true;       // float P2M in chunks of 16 right-hand sides on the matrix cores
   // (target leaf, first point) of the matrix-core L2P's point tiles, per
   // tile size (16 or 32 points: the order's l2pMfmaTile)
   int2* l2pTiles = nullptr;
@@ -2201,6 +2324,18 @@ static int upload(Exec& X, Plan& pl) {
       for (int k0 = 0; k0 < std::max(1, pl.S.nslot[c]); k0 += C) it.push_back(make_int2(c, k0));
     CKR(X.upv(X.p2mItems, it));
     X.np2mItems = (int)it.size();
+    std::vector<int2> mf, big;
+    for (int c : pl.p2mCell) {
+      const int nsl = std::max(1, pl.S.nslot[c]);
+      if (pl.S.nprt[c] <= kP2MMfmaPoints)
+        for (int k0 = 0; k0 < nsl; k0 += kP2MMfmaSlots) mf.push_back(make_int2(c, k0));
+      else
+        for (int k0 = 0; k0 < nsl; k0 += C) big.push_back(make_int2(c, k0));
+    }
+    CKR(X.upv(X.p2mMfmaItems, mf));
+    X.np2mMfma = (int)mf.size();
+    CKR(X.upv(X.p2mBigItems, big));
+    X.np2mBig = (int)big.size();
   }
   CKR(X.upv(X.l2p, pl.l2pCell)); X.nl2p = (int)pl.l2pCell.size();
   {
@@ -2446,10 +2581,30 @@ static int runTyped(Exec& X, const double2* qin, double2* pout, int parts, int n
   const bool far = (parts & 2) != 0, near = (parts & 1) != 0;
 
   if (far && X.np2mItems)
-    for (int r = 0; r < nrhs; r++)
+    for (int r = 0; r < nrhs;) {
+      if constexpr (sizeof(R) == 4) {
+        if (nrhs - r >= kP2MMfmaNR && X.p2mMfma) {
+          // leaves of <= 64 points on the matrix cores, the others per
+          // right-hand side on the vector units
+          if (X.np2mMfma)
+            hipLaunchKernelGGL((k_p2m_mfma<L>), itemGrid(X.np2mMfma), dim3(256), 0, s,
+                               X.p2mMfmaItems, X.S, X.lv, X.sx, X.sy, X.sz,
+                               (const float2*)qr(r), (size_t)ns, X.dirs, (const float2*)sig2, oc,
+                               X.kappa, (float2*)Mr(r), X.mStride, X.np2mMfma);
+          if (X.np2mBig)
+            for (int rr = r; rr < r + kP2MMfmaNR; rr++)
+              hipLaunchKernelGGL((k_p2m<R, L>), itemGrid(X.np2mBig), dim3(128), 0, s,
+                                 X.p2mBigItems, X.S, X.lv, X.sx, X.sy, X.sz, (const R2*)qr(rr),
+                                 X.dirs, sig2, oc, X.kappa, Mr(rr), X.np2mBig);
+          r += kP2MMfmaNR;
+          continue;
+        }
+      }
       hipLaunchKernelGGL((k_p2m<R, L>), itemGrid(X.np2mItems), dim3(128), 0, s, X.p2mItems, X.S,
                          X.lv, X.sx, X.sy, X.sz, (const R2*)qr(r), X.dirs, sig2, oc, X.kappa,
                          Mr(r), X.np2mItems);
+      r++;
+    }
   mark(2);
   if (far)
     for (int r = 0; r < nrhs; r++)
@@ -2863,9 +3018,12 @@ int defmm_plan_set_option(defmm_plan plan, int option, int64_t value) {
   switch (option) {
     case DEFMM_OPT_P2P_MFMA:
     case DEFMM_OPT_L2P_MFMA:
+    case DEFMM_OPT_P2M_MFMA:
       if (value != 0 && value != 1) { g_err = "the option takes 0 or 1"; return DEFMM_EARG; }
       if (X.quiesce() != dfm::OK) { g_err = "device synchronisation failed"; return DEFMM_EHIP; }
-      (option == DEFMM_OPT_P2P_MFMA ? X.p2pMfma : X.l2pMfma) = value != 0;
+      (option == DEFMM_OPT_P2P_MFMA ? X.p2pMfma
+                                    : option == DEFMM_OPT_L2P_MFMA ? X.l2pMfma : X.p2mMfma) =
+          value != 0;
       return DEFMM_OK;
   }
   g_err = "unknown option";

@@ -209,9 +209,12 @@ int defmm_plan_set_memory_limit(defmm_plan plan, int64_t bytes);
  * right-hand sides per kernel value).  DEFMM_OPT_L2P_MFMA: the same for L2P
  * (1 by default: 16 right-hand sides per pass over a leaf's slots on the
  * matrix cores; 0: one right-hand side per pass on the vector units).
+ * DEFMM_OPT_P2M_MFMA: the same for P2M (leaves of at most 64 points; larger
+ * leaves stay on the vector units).
  */
 #define DEFMM_OPT_P2P_MFMA 1
 #define DEFMM_OPT_L2P_MFMA 2
+#define DEFMM_OPT_P2M_MFMA 3
 int defmm_plan_set_option(defmm_plan plan, int option, int64_t value);
 
 /* Bytes of device memory held by the plan. */

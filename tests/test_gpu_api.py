@@ -241,3 +241,33 @@ def test_l2p_matrix_cores_match_vector_units(L):
     _say(f"  matrix cores vs vector units: worst rel L2 {d_valu:.3e}; vs FP64 {d_ref:.3e}")
     assert d_valu <= 1e-6
     assert d_ref <= 3e-6
+
+
+@pytest.mark.parametrize("L,ncrit", [(5, 97), (8, 65)])
+def test_p2m_matrix_cores_match_vector_units(L, ncrit):
+    """P2M of the mixed path on the matrix cores (16 right-hand sides, leaves
+    of at most 64 points; with Ncrit 97 the larger leaves stay on the vector
+    units in the same pass) against the vector-unit kernel, and the whole
+    pass against FP64."""
+    import defmm_amd as dm
+    from defmm_amd import workloads as wl
+    pts = wl.ellipsoid_clusters(10000, seed=17)
+    kappa = wl.kappa_for(32, pts)
+    n, nr = len(pts), 16
+    Q = np.stack([wl.charges(n, seed=950 + r) for r in range(nr)])
+    _say(f"case: P2M on matrix cores, ellipsoid N={n} 32 wavelengths L={L} Ncrit={ncrit}, "
+         f"{nr} right-hand sides")
+    A = _plan(dm, pts, L, ncrit, kappa, dm.FP64)
+    A.set_rhs(nr)
+    ref = A.apply_batch(Q)
+    A.set_precision(dm.MIXED)
+    A.set_option(dm.OPT_P2M_MFMA, 0)
+    valu = A.apply_batch(Q)
+    A.set_option(dm.OPT_P2M_MFMA, 1)
+    mfma = A.apply_batch(Q)
+    A.destroy()
+    d_valu = max(_diff(mfma[r], valu[r])[0] for r in range(nr))
+    d_ref = max(_diff(mfma[r], ref[r])[0] for r in range(nr))
+    _say(f"  matrix cores vs vector units: worst rel L2 {d_valu:.3e}; vs FP64 {d_ref:.3e}")
+    assert d_valu <= 1e-6
+    assert d_ref <= 3e-6
