@@ -91,6 +91,7 @@ def lib():
     L.defmm_plan_get_m2l.argtypes = [P, ctypes.POINTER(ctypes.c_int)]
     L.defmm_plan_set_memory_limit.argtypes = [P, ctypes.c_int64]
     L.defmm_plan_set_option.argtypes = [P, ctypes.c_int, ctypes.c_int64]
+    L.defmm_plan_autotune.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
     L.defmm_batch_create.argtypes = [ctypes.POINTER(P), ctypes.c_int, ctypes.POINTER(P),
                                      ctypes.POINTER(P), ctypes.POINTER(ctypes.c_int), ctypes.c_int,
                                      ctypes.POINTER(P)]
@@ -200,6 +201,7 @@ class IBFMM_Mat:
                                        float(kappa), int(lvMAX), int(self.precision),
                                        ctypes.byref(plan)))
         self._plan = plan
+        self._width = 1
 
     def _need_plan(self):
         if not self._plan:
@@ -229,6 +231,11 @@ class IBFMM_Mat:
         """Width of the expansion stores in right-hand sides (reallocates)."""
         self._need_plan()
         _check(lib().defmm_plan_set_rhs(self._plan, int(nrhs)))
+        self._width = int(nrhs)
+
+    def width(self):
+        """Right-hand sides the plan's stores hold (set_rhs; 1 after prcmpt)."""
+        return getattr(self, "_width", 1)
 
     def apply_batch(self, Q):
         """Evaluate P = A Q for several right-hand sides: Q (nrhs, n_src)
@@ -288,6 +295,16 @@ class IBFMM_Mat:
         matrix cores, 16 right-hand sides at a time; 1 by default)."""
         self._need_plan()
         _check(lib().defmm_plan_set_option(self._plan, int(option), int(value)))
+
+    def autotune(self, nrhs=None, reps=3):
+        """Pick the M2L storage variant and the matrix-core options by timing
+        passes of nrhs right-hand sides (default: the plan's width); returns
+        the autotuner's report."""
+        self._need_plan()
+        buf = ctypes.create_string_buffer(4096)
+        _check(lib().defmm_plan_autotune(self._plan, int(nrhs or self.width()), int(reps), buf,
+                                         len(buf)))
+        return buf.value.decode()
 
     def set_memory_limit(self, nbytes):
         """Device memory budget of later reallocations (0: none)."""

@@ -2743,6 +2743,47 @@ static int run(Exec& X, const double2* qin, double2* pout, int parts, int nrhs, 
   return OK;
 }
 
+// Unit charges for the autotuner's passes.
+__global__ void k_fill_one(double2* a, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    a[i] = make_double2(1.0, 0.0);
+}
+
+// Median time (ms) of `reps` passes of nrhs right-hand sides on the plan's
+// staging buffers (charges in qio, potentials to pio), on its own stream.
+static int timePasses(Exec& X, int nrhs, int reps, double& ms) {
+  hipEvent_t a = nullptr, b = nullptr;
+  HIPCK(hipEventCreate(&a));
+  if (hipEventCreate(&b) != hipSuccess) {
+    (void)hipEventDestroy(a);
+    g_err = "hipEventCreate failed";
+    return ERR_HIP;
+  }
+  std::vector<double> t;
+  int rc = OK;
+  // one untimed pass first (first-touch of the stores)
+  for (int k = -1; k < reps && rc == OK; k++) {
+    if (hipEventRecord(a, X.own) != hipSuccess) { rc = ERR_HIP; break; }
+    rc = run(X, X.qio, X.pio, 3, nrhs, X.own);
+    if (rc) break;
+    float f = 0.f;
+    if (hipEventRecord(b, X.own) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+        hipEventElapsedTime(&f, a, b) != hipSuccess) {
+      g_err = "autotune: event timing failed";
+      rc = ERR_HIP;
+      break;
+    }
+    if (k >= 0) t.push_back(f);
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  if (rc) return rc;
+  std::sort(t.begin(), t.end());
+  ms = t[t.size() / 2];
+  return OK;
+}
+
 // Work of one plan's pass in arbitrary units (near-field pairs plus M2L
 // pair-frequencies, per right-hand side), for balancing graph branches.
 static double passWork(const Exec& X, int nrhs) {
@@ -3028,6 +3069,79 @@ int defmm_plan_set_option(defmm_plan plan, int option, int64_t value) {
   }
   g_err = "unknown option";
   return DEFMM_EARG;
+}
+
+int defmm_plan_autotune(defmm_plan plan, int nrhs, int reps, char* report, int report_len) {
+  if (!plan) { g_err = "null plan"; return DEFMM_EARG; }
+  Exec& X = plan->x;
+  if (nrhs < 1 || nrhs > X.nr) { g_err = "nrhs must be in 1 .. the plan's width"; return DEFMM_EARG; }
+  if (reps < 1 || reps > 100) { g_err = "reps must be in 1..100"; return DEFMM_EARG; }
+  if (!X.valid) { g_err = "the plan is unusable"; return DEFMM_ESTATE; }
+  if (X.quiesce() != dfm::OK) { g_err = "device synchronisation failed"; return DEFMM_EHIP; }
+  std::string rep;
+  char buf[256];
+  auto fill = [&]() -> int {
+    hipLaunchKernelGGL(dfm::k_fill_one, dim3(dfm::zeroGrid((int64_t)nrhs * X.ns, 256)), dim3(256),
+                       0, X.own, X.qio, (int64_t)nrhs * X.ns);
+    return hipGetLastError() == hipSuccess ? dfm::OK : dfm::ERR_HIP;
+  };
+  // 1. M2L storage variant
+  {
+    struct Cand { int mode, rows; };
+    std::vector<Cand> cands = {{DEFMM_M2L_STORED, X.batchRows}, {DEFMM_M2L_BATCHED, 4096},
+                               {DEFMM_M2L_BATCHED, 16384}, {DEFMM_M2L_BATCHED, 65536}};
+    const int mode0 = X.m2lMode, rows0 = X.batchRows;
+    const bool auto0 = X.m2lAuto;
+    double best = 1e300;
+    Cand keep = {mode0, rows0};
+    for (const Cand& c : cands) {
+      X.m2lMode = c.mode;
+      X.batchRows = c.rows;
+      X.m2lAuto = false;
+      int rc = dfm::reupload(X, X.nr, X.prec, mode0, auto0, rows0);
+      if (rc == DEFMM_EMEMORY) {
+        std::snprintf(buf, sizeof buf, "m2l %s/%d: does not fit; ", c.mode == DEFMM_M2L_STORED ? "stored" : "batched", c.rows);
+        rep += buf;
+        continue;
+      }
+      if (rc) return rc;
+      double ms = 0.;
+      if ((rc = fill()) || (rc = dfm::timePasses(X, nrhs, reps, ms))) return rc;
+      std::snprintf(buf, sizeof buf, "m2l %s/%d: %.3f ms; ", c.mode == DEFMM_M2L_STORED ? "stored" : "batched", c.rows, ms);
+      rep += buf;
+      if (ms < best) { best = ms; keep = c; }
+    }
+    X.m2lMode = keep.mode;
+    X.batchRows = keep.rows;
+    X.m2lAuto = false;
+    int rc = dfm::reupload(X, X.nr, X.prec, mode0, auto0, rows0);
+    if (rc) return rc;
+    std::snprintf(buf, sizeof buf, "kept m2l %s/%d\n", keep.mode == DEFMM_M2L_STORED ? "stored" : "batched", keep.rows);
+    rep += buf;
+  }
+  // 2. matrix cores per stage (mixed precision, chunks of 16 right-hand sides)
+  if (X.prec == DEFMM_MIXED && nrhs >= dfm::kP2PMfmaNR) {
+    struct Knob { const char* name; bool* v; };
+    Knob knobs[3] = {{"p2p_mfma", &X.p2pMfma}, {"l2p_mfma", &X.l2pMfma}, {"p2m_mfma", &X.p2mMfma}};
+    for (Knob& k : knobs) {
+      double t[2] = {0., 0.};
+      for (int on = 0; on < 2; on++) {
+        *k.v = on != 0;
+        int rc = fill();
+        if (rc || (rc = dfm::timePasses(X, nrhs, reps, t[on]))) return rc;
+      }
+      *k.v = t[1] <= t[0];
+      std::snprintf(buf, sizeof buf, "%s off %.3f ms, on %.3f ms: kept %s\n", k.name, t[0], t[1],
+                    *k.v ? "on" : "off");
+      rep += buf;
+    }
+  }
+  X.evRecorded = false;
+  if (report && report_len > 0) {
+    std::strncpy(report, rep.c_str(), (size_t)report_len - 1);
+    report[report_len - 1] = 0;
+  }
+  return DEFMM_OK;
 }
 
 int defmm_plan_set_memory_limit(defmm_plan plan, int64_t bytes) {

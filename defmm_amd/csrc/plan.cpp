@@ -445,8 +445,12 @@ int validatePlan(const Plan& pl, std::string& msg);
 int buildPlan(Plan& pl, const double* src, int64_t ns, const double* trg, int64_t nt, int L,
               int ncrit, double kappa, std::string& msg) {
   const int lvmax = ncrit < 0 ? -ncrit : -1;  // negative Ncrit: fixed depth
-  if (!src || !trg || ns <= 0 || nt <= 0 || L < 2 || L > 12 || (ncrit == 0) || kappa <= 0 ||
-      ns > (int64_t)2000000000 || nt > (int64_t)2000000000) {
+  // kappa = 0 is the Laplace kernel 1/(4 pi r) (the reference's
+  // thetis::kernels::Laplace, include/thetis/thetis_kernels.hpp:35): every
+  // level is low-frequency and M2L starts at level 2, as in the reference's
+  // initFrequency (include/tables/frequency.hpp:43-129)
+  if (!src || !trg || ns <= 0 || nt <= 0 || L < 2 || L > 12 || (ncrit == 0) ||
+      !(kappa >= 0) || ns > (int64_t)2000000000 || nt > (int64_t)2000000000) {
     msg = "invalid argument";
     return ERR_ARG;
   }

@@ -52,7 +52,9 @@ typedef struct defmm_plan_s* defmm_plan;
  * hold n*3 doubles (x, y, z per point, input order).  order = L (>= 2),
  * ncrit: a cell is split while it holds >= ncrit points; lvmax >= 0 instead
  * splits every cell down to depth lvmax (the reference's fixed-depth build,
- * tree.hpp:104-132), -1 to use ncrit.  precision: DEFMM_FP64 or DEFMM_MIXED
+ * tree.hpp:104-132), -1 to use ncrit.  kappa >= 0: the wavenumber; 0 gives
+ * the Laplace kernel 1/(4*pi*r) (the reference's thetis::kernels::Laplace)
+ * with every level low-frequency.  precision: DEFMM_FP64 or DEFMM_MIXED
  * (device storage is allocated for that arithmetic).  Device: the current
  * HIP device.
  */
@@ -216,6 +218,22 @@ int defmm_plan_set_memory_limit(defmm_plan plan, int64_t bytes);
 #define DEFMM_OPT_L2P_MFMA 2
 #define DEFMM_OPT_P2M_MFMA 3
 int defmm_plan_set_option(defmm_plan plan, int option, int64_t value);
+
+/*
+ * Autotuner of a plan's execution knobs for passes of nrhs right-hand sides
+ * (1 .. the plan's width), in the plan's current precision.  The pass is run
+ * on the plan's own staging buffers with unit charges, `reps` timed passes
+ * per candidate (median), one knob at a time with the others fixed, and the
+ * fastest value is kept:
+ *   1. M2L storage: stored Fourier rows (skipped when they do not fit) or
+ *      the batched variant with a workspace of 4,096, 16,384 or 65,536 rows;
+ *   2. in mixed precision with nrhs >= 16: matrix cores on or off for P2P,
+ *      L2P and P2M (DEFMM_OPT_*_MFMA).
+ * Writes a text report of every candidate's time and the kept values to
+ * report (at most report_len bytes, NUL-terminated; report may be NULL).
+ * Waits for the plan's previous executions; synchronous.
+ */
+int defmm_plan_autotune(defmm_plan plan, int nrhs, int reps, char* report, int report_len);
 
 /* Bytes of device memory held by the plan. */
 int defmm_plan_device_bytes(defmm_plan plan, int64_t* bytes);

@@ -271,3 +271,28 @@ def test_p2m_matrix_cores_match_vector_units(L, ncrit):
     _say(f"  matrix cores vs vector units: worst rel L2 {d_valu:.3e}; vs FP64 {d_ref:.3e}")
     assert d_valu <= 1e-6
     assert d_ref <= 3e-6
+
+
+def test_autotune_keeps_results():
+    """The autotuner times the M2L storage variants and the matrix-core
+    options on the plan's own buffers, keeps the fastest, and the plan's
+    results stay those of the mixed path (within float roundoff of the
+    untuned plan, which runs other kernels for some stages)."""
+    import defmm_amd as dm
+    from defmm_amd import workloads as wl
+    pts = wl.cube_edges(6000, seed=18)
+    kappa = wl.kappa_for(16, pts)
+    n, nr = len(pts), 16
+    Q = np.stack([wl.charges(n, seed=990 + r) for r in range(nr)])
+    _say(f"case: autotune, edge-refined cube N={n} 16 wavelengths L=6, {nr} right-hand sides")
+    A = _plan(dm, pts, 6, 65, kappa, dm.MIXED)
+    A.set_rhs(nr)
+    before = A.apply_batch(Q)
+    rep = A.autotune(nr, reps=2)
+    _say("  " + rep.replace("\n", "\n  "))
+    after = A.apply_batch(Q)
+    A.destroy()
+    assert "kept m2l" in rep and "p2p_mfma" in rep and "l2p_mfma" in rep and "p2m_mfma" in rep
+    d = max(_diff(after[r], before[r])[0] for r in range(nr))
+    _say(f"  after vs before: worst rel L2 {d:.3e}")
+    assert d <= 1e-6
