@@ -30,13 +30,15 @@ wavelengths, order 5): FP64 and mixed at full size; at this frequency order 5
 gives 7e-3, and the order that reaches 1e-6 does not fit in one GPU's memory
 (see the design notes), so it reports throughput and error as measured.
 
-Roofline: the dominant kernel of the value pass.  Compulsory HBM bytes of
-the fused M2L+F2L kernel = Fourier rows read once per right-hand side + kernel
-table rows read once per launch + local expansions written once per
-right-hand side + the group lists; divided by the stage's time measured with
-HIP events on the launch stream.  `traffic` is the measured FETCH_SIZE +
-WRITE_SIZE per launch from a rocprofv3 counter run (profiles/), scaled by a
-copy kernel of known size.
+Roofline: the dominant stage of the value pass, the fused M2L+F2L (with, in
+the batched M2L variant, the M2F launches that refill its workspace).
+Compulsory HBM bytes = source expansions (batched: M rows; stored: Fourier
+rows) read once per right-hand side + kernel table rows read once per
+launch + local expansions written once per right-hand side + the group
+lists; divided by the stage's time measured with HIP events on the launch
+stream.  `traffic` is the stage's FETCH_SIZE + WRITE_SIZE per pass from
+rocprofv3 counter runs of the same variant (profiles/m2l_traffic_*.txt,
+written by profile_summary.py), scaled by a copy kernel of known size.
 
 CPU baseline: the oracle (serial FP64 restatement of the reference) at order
 8 on a reduced twin of the workload (8,000 cube points at the same points per
@@ -85,18 +87,20 @@ def m2l_nr_per_launch(L, mixed):
     return w
 
 
-def m2l_compulsory_bytes(st, L, nrhs, mixed):
-    """Compulsory HBM bytes of the M2L+F2L stage (stored Fourier rows) for
-    nrhs right-hand sides: every F row read once per right-hand side, every
-    table row once per launch, every local expansion written once per
-    right-hand side, and the group lists once per launch."""
+def m2l_compulsory_bytes(st, L, nrhs, mixed, batched=False):
+    """Compulsory HBM bytes of the M2L+F2L stage for nrhs right-hand sides:
+    stored variant: every F row read once per right-hand side; batched
+    variant: every source M row read once per right-hand side (its F row is
+    made by the batch's M2F and read back while cache-resident); both: every
+    table row once per launch over all groups, every local expansion written
+    once per right-hand side, and the group lists once per launch."""
     PP, LL = (2 * L - 1) ** 3, L ** 3
     s = 8 if mixed else 16
     g = 1 if L >= 8 else 3 if L == 7 else 4
     nr = m2l_nr_per_launch(L, mixed)
     launches = -(-nrhs // nr)
     lists = st["m2l_group_entries"] * (4 + 4 + 4 * g) + st["m2l_groups"] * (4 + 4 * g)
-    per_rhs = st["m2f_slots"] * PP * s + st["m2l_targets"] * LL * s
+    per_rhs = st["m2f_slots"] * (LL if batched else PP) * s + st["m2l_targets"] * LL * s
     per_launch = st["m2l_table_rows"] * PP * s + lists
     return nrhs * per_rhs + launches * per_launch, launches
 
@@ -222,18 +226,31 @@ class Runner:
         st, L = self.st, self.L
         dom = max(["m2l_f2l", "p2p"], key=lambda k: stage_ms[k])
         t = stage_ms[dom] * 1e-3
-        if dom == "m2l_f2l" and self.A.m2l_mode() == self.dm.M2L_STORED:
-            comp, launches = m2l_compulsory_bytes(st, L, nrhs, mixed)
+        mode = self.A.m2l_mode()
+        if dom == "m2l_f2l" and mode in (self.dm.M2L_STORED, self.dm.M2L_BATCHED):
+            batched = mode == self.dm.M2L_BATCHED
+            comp, launches = m2l_compulsory_bytes(st, L, nrhs, mixed, batched)
             achieved = comp / t / 1e9
-            pair_bytes = (st["m2l_entries"] + st["m2l_targets"]) * (2 * L - 1) ** 3 * \
-                (8 if mixed else 16) * nrhs
-            return {"kernel": "k_m2l (fused M2L Hadamard + F2L)", "bound": "hbm",
-                    "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                    "frac": achieved / PEAK_HBM_GBS, "traffic": None,
-                    "compulsory_bytes_per_launch": comp / launches, "launches": launches,
-                    "avg_ms_per_launch": stage_ms[dom] / launches,
-                    "pair_bytes_effective_GBs": pair_bytes / t / 1e9,
-                    "dtype": "f32" if mixed else "f64"}
+            kname = "k_m2l_col" if L >= 8 else "k_m2l"
+            # the bytes the Hadamard reads from L2 / MALL / HBM: an F row and
+            # a table row per pair (the table row once per launch's right-hand
+            # sides)
+            s_c = 8 if mixed else 16
+            nr = m2l_nr_per_launch(L, mixed)
+            pair_bytes = st["m2l_entries"] * (2 * L - 1) ** 3 * s_c * (nrhs + -(-nrhs // nr))
+            out = {"kernel": f"{kname} (fused M2L Hadamard + F2L)" +
+                             (f", with k_m2f per batch of F rows ({st['m2l_batches']} batches)"
+                              if batched else ""),
+                   "m2l_variant": "batched" if batched else "stored",
+                   "bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                   "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                   "compulsory_bytes_per_pass": comp, "launches": launches,
+                   "avg_ms_per_launch": stage_ms[dom] / launches,
+                   "pair_bytes_read_GBs": pair_bytes / t / 1e9,
+                   "dtype": "f32" if mixed else "f64"}
+            if not batched:
+                out["compulsory_bytes_per_launch"] = comp / launches
+            return out
         flops = st["p2p_pairs"] * P2P_FLOPS_PER_PAIR * nrhs
         peak = PEAK_FP32_TFLOPS if mixed else PEAK_FP64_TFLOPS
         achieved = flops / t / 1e12
@@ -379,12 +396,13 @@ def multi_rhs_section(name, nrhs, width, steps, warmup, samples=200):
     return out
 
 
-def read_traffic(kernel, workload):
+def read_traffic(kernel, workload, key="traffic_bytes_per_launch"):
+    """A measured traffic figure from a profile summary under profiles/."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_traffic_{workload}.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.startswith(key + ":"):
             return float(line.split(":")[1])
     return None
 
@@ -401,6 +419,11 @@ def main():
     ap.add_argument("--scale-workload", default="sphere1m_128l",
                     help="full-size workload timed in FP64 and mixed ('none' to skip)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="time every pass with the library's default settings")
+    ap.add_argument("--m2l-rows", type=int, default=0,
+                    help="time the mixed batched pass with the batched M2L variant and this many "
+                         "workspace rows instead of autotuning it (profiling runs)")
     ap.add_argument("--batch-instances", type=int, default=48,
                     help="instances of the uneven instance batch (0 to skip)")
     ap.add_argument("--batch-sizes", default="10000,100000",
@@ -425,16 +448,36 @@ def main():
     import defmm_amd as dm
 
     R = Runner(args.workload, args.samples, args.steps, args.warmup)
+    tunes = {}
+
+    def tune(tag, nr):
+        if args.no_autotune:
+            return
+        t0 = time.time()
+        tunes[tag] = R.A.autotune(nr, reps=3)
+        log(f"autotune {tag} ({time.time() - t0:.1f} s):\n{tunes[tag]}")
+
+    tune("fp64", 1)
     ms64, e64, einf64, st64 = R.timed(1, "fp64")
     roof64 = R.roofline(st64, 1, False)
+    v64_variant = ["on the fly", "stored", "batched"][R.A.m2l_mode()]
     R.A.set_precision(dm.MIXED)
+    tune("mixed", 1)
     msmx, emx, einfmx, stmx = R.timed(1, "mixed")
     R.A.set_rhs(args.nrhs)
+    if args.m2l_rows > 0:
+        R.A.set_m2l(dm.M2L_BATCHED, batch_rows=args.m2l_rows)
+    else:
+        tune("mixed_batched", args.nrhs)
     msb, eb, einfb, stb = R.timed(args.nrhs, f"mixed x{args.nrhs}")
     roof = R.roofline(stb, args.nrhs, True)
-    roof["traffic"] = read_traffic("m2l_mixed", args.workload)
-    if roof["traffic"]:
-        roof["traffic_over_compulsory"] = roof["traffic"] / roof["compulsory_bytes_per_launch"]
+    if roof.get("m2l_variant"):
+        # HBM bytes of the M2L stage per pass, measured with rocprofv3 counters
+        # in the same variant (profiles/m2l_traffic_<workload>.txt)
+        roof["traffic"] = read_traffic("m2l", args.workload,
+                                       f"{roof['m2l_variant']}_f32_x{args.nrhs}_bytes_per_pass")
+        if roof["traffic"]:
+            roof["traffic_over_compulsory"] = roof["traffic"] / roof["compulsory_bytes_per_pass"]
     variant = ["on the fly", "stored", "batched"][R.A.m2l_mode()]
     n = R.n
     v64 = n / (ms64 * 1e-3) / 1e6
@@ -460,9 +503,12 @@ def main():
         S = Runner(args.scale_workload, 2000, max(1, min(args.steps, 5)), 1)
         s64, se64, _, sst64 = S.timed(1, "fp64")
         sroof = S.roofline(sst64, 1, False)
-        sroof["traffic"] = read_traffic("m2l_stored", args.scale_workload)
-        if sroof["traffic"] and "compulsory_bytes_per_launch" in sroof:
-            sroof["traffic_over_compulsory"] = sroof["traffic"] / sroof["compulsory_bytes_per_launch"]
+        if sroof.get("m2l_variant"):
+            sroof["traffic"] = read_traffic("m2l", args.scale_workload,
+                                            f"{sroof['m2l_variant']}_f64_x1_bytes_per_pass")
+            if sroof["traffic"]:
+                sroof["traffic_over_compulsory"] = (sroof["traffic"] /
+                                                    sroof["compulsory_bytes_per_pass"])
         S.A.set_precision(dm.MIXED)
         smx, semx, _, sstmx = S.timed(1, "mixed")
         scale = {"workload": S.describe(), "setup_s": S.setup_s,
@@ -499,19 +545,22 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"{R.describe()}, {args.nrhs} RHS, mixed precision (float "
-                               f"expansions, double phase reduction), stored M2L",
+                               f"expansions, double phase reduction), {variant} M2L",
                    "n_points": n, "kappa": R.kappa, "order": R.L, "ncrit": R.ncrit,
                    "nrhs": args.nrhs, "parallelism": "single GPU"},
         "accuracy": {"rel_l2_err": eb, "rel_linf_err": einfb, "targets": args.samples,
                      "gate_1e-6_met": gate_ok,
                      "reference": "FP64 direct sum (GPU) on sampled targets"},
         "fp64": {"value": v64, "ms_per_step": ms64, "rel_l2_err": e64, "rel_linf_err": einf64,
-                 "stage_ms": st64, "roofline": roof64},
+                 "stage_ms": st64, "roofline": roof64, "m2l_variant": v64_variant,
+                 "autotune": tunes.get("fp64")},
         "mixed": {"value": vmx, "ms_per_step": msmx, "rel_l2_err": emx, "rel_linf_err": einfmx,
-                  "speedup_vs_fp64": v64 and vmx / v64, "stage_ms": stmx},
+                  "speedup_vs_fp64": v64 and vmx / v64, "stage_ms": stmx,
+                  "autotune": tunes.get("mixed")},
         "mixed_batched": {"value": value, "nrhs": args.nrhs, "ms_per_step": msb,
                           "rel_l2_err_first_rhs": eb, "speedup_vs_fp64": value / v64,
-                          "stage_ms": stb, "m2l_variant": variant},
+                          "stage_ms": stb, "m2l_variant": variant,
+                          "autotune": tunes.get("mixed_batched")},
         "speedup_mixed_batched_vs_fp64": value / v64,
         "roofline": roof,
         "cpu_baseline": cpu,
