@@ -18,6 +18,7 @@
 #include "../../include/defmm_amd.h"
 #include "errors.hpp"
 #include "grid.hpp"
+#include "dft.hpp"
 #include "kernels.hpp"
 #include "plan.hpp"
 
@@ -589,7 +590,8 @@ constexpr int m2fRows() {
 // Each pass gives one lane a whole line along its axis: the lane reads the
 // line's L inputs once and forms all P outputs, with twiddles whose indices
 // are compile-time constants (held in registers, not read from LDS); the last
-// pass stores F straight from registers.
+// pass stores F straight from registers.  For P = 15 (order 8) a line is the
+// prime-factor 15-point DFT of dft.hpp (168 instead of 480 operations).
 template <typename R, int L>
 __global__ __launch_bounds__(256) void k_m2f(const int* __restrict__ m2fSlot,
                                              const typename C2T<R>::T* __restrict__ M,
@@ -621,12 +623,21 @@ __global__ __launch_bounds__(256) void k_m2f(const int* __restrict__ m2fSlot,
     R2 v[L];
 #pragma unroll
     for (int j = 0; j < L; j++) v[j] = Mr[j];
+    if constexpr (P == 15) {
+      R2 y[P];
 #pragma unroll
-    for (int k2 = 0; k2 < P; k2++) {
-      R2 t = c2<R>(0, 0);
+      for (int j = 0; j < L; j++) y[j] = v[j];
+      dft15<-1, L>(y);
 #pragma unroll
-      for (int j = 0; j < L; j++) mac(t, v[j], (j * k2) % P);
-      T1[k][r * P + k2] = t;
+      for (int k2 = 0; k2 < P; k2++) T1[k][r * P + k2] = y[k2];
+    } else {
+#pragma unroll
+      for (int k2 = 0; k2 < P; k2++) {
+        R2 t = c2<R>(0, 0);
+#pragma unroll
+        for (int j = 0; j < L; j++) mac(t, v[j], (j * k2) % P);
+        T1[k][r * P + k2] = t;
+      }
     }
   }
   __syncthreads();
@@ -637,12 +648,21 @@ __global__ __launch_bounds__(256) void k_m2f(const int* __restrict__ m2fSlot,
     R2 v[L];
 #pragma unroll
     for (int j = 0; j < L; j++) v[j] = T1[k][(i0 * L + j) * P + k2];
+    if constexpr (P == 15) {
+      R2 y[P];
 #pragma unroll
-    for (int k1 = 0; k1 < P; k1++) {
-      R2 t = c2<R>(0, 0);
+      for (int j = 0; j < L; j++) y[j] = v[j];
+      dft15<-1, L>(y);
 #pragma unroll
-      for (int j = 0; j < L; j++) mac(t, v[j], (j * k1) % P);
-      T2[k][(i0 * P + k1) * P + k2] = t;
+      for (int k1 = 0; k1 < P; k1++) T2[k][(i0 * P + k1) * P + k2] = y[k1];
+    } else {
+#pragma unroll
+      for (int k1 = 0; k1 < P; k1++) {
+        R2 t = c2<R>(0, 0);
+#pragma unroll
+        for (int j = 0; j < L; j++) mac(t, v[j], (j * k1) % P);
+        T2[k][(i0 * P + k1) * P + k2] = t;
+      }
     }
   }
   __syncthreads();
@@ -653,12 +673,21 @@ __global__ __launch_bounds__(256) void k_m2f(const int* __restrict__ m2fSlot,
 #pragma unroll
     for (int j = 0; j < L; j++) v[j] = T2[k][j * P * P + r];
     R2* Fr = F + (size_t)(row0 + k) * PP + r;
+    if constexpr (P == 15) {
+      R2 y[P];
 #pragma unroll
-    for (int k0 = 0; k0 < P; k0++) {
-      R2 t = c2<R>(0, 0);
+      for (int j = 0; j < L; j++) y[j] = v[j];
+      dft15<-1, L>(y);
 #pragma unroll
-      for (int j = 0; j < L; j++) mac(t, v[j], (j * k0) % P);
-      Fr[k0 * P * P] = t;
+      for (int k0 = 0; k0 < P; k0++) Fr[k0 * P * P] = y[k0];
+    } else {
+#pragma unroll
+      for (int k0 = 0; k0 < P; k0++) {
+        R2 t = c2<R>(0, 0);
+#pragma unroll
+        for (int j = 0; j < L; j++) mac(t, v[j], (j * k0) % P);
+        Fr[k0 * P * P] = t;
+      }
     }
   }
 }
@@ -722,6 +751,17 @@ constexpr int m2lMaxNR() {
   constexpr int w = cap >= 8 ? 8 : cap >= 4 ? 4 : cap >= 2 ? 2 : 1;
   return (sizeof(R) == 8 && L >= 7) ? 1 : (L == 7 && w > 2) ? 2 : w;
 }
+static int m2lMaxNRAt(int prec, int L) {
+  switch (L) {
+#define DFM_NR_CASE(l) \
+    case l: return prec == DEFMM_FP64 ? m2lMaxNR<double, l>() : m2lMaxNR<float, l>();
+    DFM_NR_CASE(2) DFM_NR_CASE(3) DFM_NR_CASE(4) DFM_NR_CASE(5) DFM_NR_CASE(6)
+    DFM_NR_CASE(7) DFM_NR_CASE(8) DFM_NR_CASE(9) DFM_NR_CASE(10)
+#undef DFM_NR_CASE
+    default: return 1;
+  }
+}
+
 // Waves per SIMD the compiler is asked to fit: at L = 7 (768-thread blocks)
 // float with two right-hand sides took 105 VGPRs, one block per CU; 6 waves
 // per SIMD (two blocks) fit without spilling in groups of 3 targets (in
@@ -941,20 +981,52 @@ __global__ __launch_bounds__(M2LShape<L>::TB) void k_m2l_col(
     if (rr > 0) __syncthreads();  // T1 / T2 of the previous right-hand side consumed
     // axis 0 in registers: T1[k0 * P^2 + col] = sum_w0 J[w0, col] e^{+2 pi i w0 k0 / P}
     if (tid < P2) {
+      if constexpr (P == 15) {
+        R2 y[P];
 #pragma unroll
-      for (int k0 = 0; k0 < L; k0++) {
-        R2 s = c2<R>(0, 0);
+        for (int m = 0; m < P; m++) y[m] = acc[m][rr];
+        dft15<1, P>(y);
 #pragma unroll
-        for (int m = 0; m < P; m++) {
-          const int q = (m * k0) % P;
-          const R2 v = acc[m][rr];
-          s.x += v.x * wc[q] - v.y * ws[q];
-          s.y += v.y * wc[q] + v.x * ws[q];
+        for (int k0 = 0; k0 < L; k0++) T1[k0 * P2 + tid] = y[k0];
+      } else {
+#pragma unroll
+        for (int k0 = 0; k0 < L; k0++) {
+          R2 s = c2<R>(0, 0);
+#pragma unroll
+          for (int m = 0; m < P; m++) {
+            const int q = (m * k0) % P;
+            const R2 v = acc[m][rr];
+            s.x += v.x * wc[q] - v.y * ws[q];
+            s.y += v.y * wc[q] + v.x * ws[q];
+          }
+          T1[k0 * P2 + tid] = s;
         }
-        T1[k0 * P2 + tid] = s;
       }
     }
     __syncthreads();
+    if constexpr (P == 15) {
+      // axis 1, a lane per line (k0, w2): T2[(k0*L + k1)*P + w2], k1 < L
+      for (int x = tid; x < L * P; x += TB) {
+        const int k0 = x / P, v2 = x - k0 * P;
+        R2 y[P];
+#pragma unroll
+        for (int v1 = 0; v1 < P; v1++) y[v1] = T1[(k0 * P + v1) * P + v2];
+        dft15<1, P>(y);
+#pragma unroll
+        for (int k1 = 0; k1 < L; k1++) T2[(k0 * L + k1) * P + v2] = y[k1];
+      }
+      __syncthreads();
+      // axis 2 and scaling, a lane per line (k0, k1): L[(k0*L + k1)*L + k2]
+      for (int x = tid; x < L * L; x += TB) {
+        R2 y[P];
+#pragma unroll
+        for (int v2 = 0; v2 < P; v2++) y[v2] = T2[x * P + v2];
+        dft15<1, P>(y);
+#pragma unroll
+        for (int k2 = 0; k2 < L; k2++) Lr[x * L + k2] = c2<R>(y[k2].x * inv, y[k2].y * inv);
+      }
+      continue;
+    }
     // axis 1: T2[(k0*L + k1)*P + w2], k1 < L
     for (int x = tid; x < L * L * P; x += TB) {
       const int v2 = x % P, k1 = (x / P) % L, k0 = x / (L * P);
@@ -2181,8 +2253,10 @@ static int uploadTyped(Exec& X) {
       CKR(buildBatches(X, local, rows));
       CKR(X.upv(X.gSrc, local));
       CKR(X.upv(X.bRows, rows));
+      // the workspace holds the right-hand sides of one M2L launch
       X.fStride = (size_t)X.wsRows * X.PP;
-      CKR(X.alloc(X.F, nr * X.fStride * sizeof(R2)));
+      const size_t nw = std::min(nr, (size_t)m2lMaxNRAt(X.prec, X.L));
+      CKR(X.alloc(X.F, nw * X.fStride * sizeof(R2)));
     }
   }
   CKR(X.alloc(X.q, nr * (size_t)X.ns * sizeof(R2)));
@@ -2489,10 +2563,20 @@ static void launchM2L(Exec& X, hipStream_t s, int g0, int ng, const typename C2T
                        X.fStride, (const R2*)X.U, Lb, X.lStride, oc.twc, oc.tws, ng);
 }
 
+// Right-hand sides of the next M2L launch when n are left.
 template <typename R, int L>
-static int m2lChunk(Exec& X, hipStream_t s, int g0, int ng, int r0, int n, const OpConsts<R>& oc) {
+constexpr int m2lChunkWidth(int n) {
+  constexpr int mx = m2lMaxNR<R, L>();
+  return (mx >= 8 && n >= 8) ? 8 : (mx >= 4 && n >= 4) ? 4 : (mx >= 2 && n >= 2) ? 2 : 1;
+}
+
+// M2L of groups g0 .. g0 + ng - 1 for right-hand sides r0 .. r0 + n' - 1
+// (n' = m2lChunkWidth(n), returned); their F rows start at F (right-hand
+// side r0 + i at F + i * fStride).
+template <typename R, int L>
+static int m2lChunk(Exec& X, hipStream_t s, int g0, int ng, int r0, int n, const OpConsts<R>& oc,
+                    const typename C2T<R>::T* F) {
   typedef typename C2T<R>::T R2;
-  const R2* F = (const R2*)X.F + (size_t)r0 * X.fStride;
   R2* Lb = (R2*)X.Lb + (size_t)r0 * X.lStride;
   constexpr int mx = m2lMaxNR<R, L>();
   if (mx >= 8 && n >= 8) { launchM2L<R, L, (mx >= 8 ? 8 : 1)>(X, s, g0, ng, F, Lb, oc); return 8; }
@@ -2621,7 +2705,8 @@ static int runTyped(Exec& X, const double2* qin, double2* pout, int parts, int n
   if (far && X.nGroups) {
     if (stored) {
       // L (aliasing M) is written for every target slot from here on
-      for (int r0 = 0; r0 < nrhs;) r0 += m2lChunk<R, L>(X, s, 0, X.nGroups, r0, nrhs - r0, oc);
+      for (int r0 = 0; r0 < nrhs;)
+        r0 += m2lChunk<R, L>(X, s, 0, X.nGroups, r0, nrhs - r0, oc, Fr(r0));
     } else if (X.m2lMode == DEFMM_M2L_ON_THE_FLY) {
       if constexpr (!M2LShape<L>::kColumns) {
         for (int r = 0; r < nrhs; r++)
@@ -2633,17 +2718,24 @@ static int runTyped(Exec& X, const double2* qin, double2* pout, int parts, int n
         return ERR_ARG;
       }
     } else {
-      // per batch: its F rows into the workspace, then its groups' M2L
-      // (stream order separates a batch's M2L from the next batch's M2F)
+      // per batch of groups and per chunk of right-hand sides of one M2L
+      // launch: the batch's F rows of the chunk into the workspace, then the
+      // batch's M2L over them.  The workspace holds one chunk, so its rows
+      // are read back while still in the caches, and the next chunk
+      // overwrites them there.  (Stream order separates an M2L launch from
+      // the next M2F into the same rows.)
       const size_t nb = X.hBatchGroupBeg.size() - 1;
       for (size_t b = 0; b < nb; b++) {
         const int g0 = X.hBatchGroupBeg[b], g1 = X.hBatchGroupBeg[b + 1];
         const int w0 = X.hBatchRowBeg[b], w1 = X.hBatchRowBeg[b + 1];
-        for (int r = 0; r < nrhs; r++)
-          hipLaunchKernelGGL((k_m2f<R, L>), itemGrid((w1 - w0 + m2fRows<R, L>() - 1) / m2fRows<R, L>()), dim3(256),
-                             0, s, X.bRows + w0,
-                             (const R2*)Mr(r), Fr(r), oc.twc, oc.tws, w1 - w0);
-        for (int r0 = 0; r0 < nrhs;) r0 += m2lChunk<R, L>(X, s, g0, g1 - g0, r0, nrhs - r0, oc);
+        for (int r0 = 0; r0 < nrhs;) {
+          const int c = m2lChunkWidth<R, L>(nrhs - r0);
+          for (int r = r0; r < r0 + c; r++)
+            hipLaunchKernelGGL((k_m2f<R, L>), itemGrid((w1 - w0 + m2fRows<R, L>() - 1) / m2fRows<R, L>()), dim3(256),
+                               0, s, X.bRows + w0,
+                               (const R2*)Mr(r), Fr(r - r0), oc.twc, oc.tws, w1 - w0);
+          r0 += m2lChunk<R, L>(X, s, g0, g1 - g0, r0, c, oc, Fr(0));
+        }
       }
     }
   }
@@ -3088,7 +3180,8 @@ int defmm_plan_autotune(defmm_plan plan, int nrhs, int reps, char* report, int r
   // 1. M2L storage variant
   {
     struct Cand { int mode, rows; };
-    std::vector<Cand> cands = {{DEFMM_M2L_STORED, X.batchRows}, {DEFMM_M2L_BATCHED, 4096},
+    std::vector<Cand> cands = {{DEFMM_M2L_STORED, X.batchRows}, {DEFMM_M2L_BATCHED, 1024},
+                               {DEFMM_M2L_BATCHED, 2048}, {DEFMM_M2L_BATCHED, 4096},
                                {DEFMM_M2L_BATCHED, 16384}, {DEFMM_M2L_BATCHED, 65536}};
     const int mode0 = X.m2lMode, rows0 = X.batchRows;
     const bool auto0 = X.m2lAuto;

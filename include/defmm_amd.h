@@ -172,9 +172,10 @@ int defmm_plan_set_precision(defmm_plan plan, int precision);
  *    store.  Memory: (2L-1)^3 complex per source slot on top of M.  Fastest.
  *  DEFMM_M2L_BATCHED: the M2L targets, in (level, direction, cell) order,
  *    are cut into batches whose source Fourier rows fit a workspace of
- *    `rows` rows (defmm_plan_set_m2l_batch_rows); per batch one M2F launch
- *    fills the workspace and one M2L launch reads it while it is still
- *    cache-resident.  Memory: M and L stores plus the workspace.
+ *    `rows` rows (defmm_plan_set_m2l_batch_rows); per batch and per chunk
+ *    of right-hand sides of one M2L launch (1 - 8), M2F fills the workspace
+ *    and the M2L launch reads it while it is still cache-resident.  Memory:
+ *    M and L stores plus the workspace of one chunk.
  *  DEFMM_M2L_ON_THE_FLY: M2L recomputes each F row from M inside the
  *    kernel; no F store.  Orders up to 7.
  */
@@ -226,7 +227,8 @@ int defmm_plan_set_option(defmm_plan plan, int option, int64_t value);
  * per candidate (median), one knob at a time with the others fixed, and the
  * fastest value is kept:
  *   1. M2L storage: stored Fourier rows (skipped when they do not fit) or
- *      the batched variant with a workspace of 4,096, 16,384 or 65,536 rows;
+ *      the batched variant with a workspace of 1,024, 2,048, 4,096, 16,384
+ *      or 65,536 rows;
  *   2. in mixed precision with nrhs >= 16: matrix cores on or off for P2P,
  *      L2P and P2M (DEFMM_OPT_*_MFMA).
  * Writes a text report of every candidate's time and the kept values to
