@@ -263,8 +263,9 @@ class IBFMM_Mat:
         self.precision = precision
 
     def set_m2l(self, mode, batch_rows=None):
-        """M2L variant: M2L_BATCHED (default), M2L_STORED or M2L_ON_THE_FLY;
-        batch_rows sets the batched variant's workspace rows."""
+        """M2L variant: M2L_STORED (chosen at creation when it fits in
+        memory), M2L_BATCHED or M2L_ON_THE_FLY; batch_rows sets the batched
+        variant's workspace rows."""
         self._need_plan()
         if batch_rows is not None:
             _check(lib().defmm_plan_set_m2l_batch_rows(self._plan, int(batch_rows)))
@@ -291,8 +292,9 @@ class IBFMM_Mat:
         return dict(zip(STAGE_NAMES, ms.tolist()))
 
     def set_option(self, option, value):
-        """Execution option (OPT_P2P_MFMA: near field of the mixed path on the
-        matrix cores, 16 right-hand sides at a time; 1 by default)."""
+        """Execution option: OPT_P2P_MFMA, OPT_L2P_MFMA, OPT_P2M_MFMA (0 or 1:
+        the stage of the mixed path on the matrix cores, 16 right-hand sides
+        at a time; 1 by default)."""
         self._need_plan()
         _check(lib().defmm_plan_set_option(self._plan, int(option), int(value)))
 
