@@ -33,6 +33,7 @@ M2L_BATCHED = 2
 OPT_P2P_MFMA = 1
 OPT_L2P_MFMA = 2
 OPT_P2M_MFMA = 3
+OPT_M2L_SLICED = 4
 
 STAT_NAMES = [
     "p2p_calls", "p2p_pairs", "hf_m2l", "lf_m2l", "src_slots", "trg_slots", "hf_tables",

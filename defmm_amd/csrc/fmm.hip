@@ -779,7 +779,13 @@ __device__ __forceinline__ size_t tableRow(int code) {
 
 // NR right-hand sides per launch: right-hand side r reads F + r * fStride
 // and writes Lout + r * lStride; each table value is loaded once for all.
-template <typename R, int L, int NR>
+// SL (sliced): the entries are swept once per frequency slice m (a thread's
+// m-th frequency) instead of once for all NW slices, so at any time a work
+// group reads 1/NW of each of its rows: the rows that neighbouring groups
+// share stay in L2 longer.  Two entries' loads are issued together.  Each
+// accumulator sums the same terms in the same order with the same fused
+// multiply-adds (cmacFma): results are identical.
+template <typename R, int L, int NR, bool SL>
 __global__ __launch_bounds__((M2LShapeN<L, NR>::TB))
 __attribute__((amdgpu_waves_per_eu(m2lWavesPerEU<R, L, NR>(), 8))) void k_m2l(
     const int* __restrict__ groupBeg, const int* __restrict__ groupTarget,
@@ -818,6 +824,44 @@ __attribute__((amdgpu_waves_per_eu(m2lWavesPerEU<R, L, NR>(), 8))) void k_m2l(
 #pragma unroll
       for (int r = 0; r < NR; r++) acc[g][m][r] = c2<R>(0, 0);
   const int e0 = groupBeg[grp], e1 = groupBeg[grp + 1];
+  if constexpr (SL) {
+    // one slice m at a time; entries e, e + 1 loaded together
+    auto ld = [&](int e, int m, R2 (&f)[NR], R2 (&u)[G], int& mask) {
+      const R2* Fr = F + (size_t)gSrcF[e] * PP + wl[m];
+      mask = gMask[e];
+#pragma unroll
+      for (int r = 0; r < NR; r++) f[r] = Fr[r * fStride];
+#pragma unroll
+      for (int g = 0; g < G; g++)
+        if ((mask >> g) & 1) u[g] = U[tableRow<PP>(gCode[(size_t)e * G + g]) + wl[m]];
+    };
+    auto mac = [&](int m, const R2 (&f)[NR], const R2 (&u)[G], int mask) {
+#pragma unroll
+      for (int g = 0; g < G; g++) {
+        if (!((mask >> g) & 1)) continue;
+#pragma unroll
+        for (int r = 0; r < NR; r++) cmacFma(acc[g][m][r], u[g], f[r]);
+      }
+    };
+#pragma unroll
+    for (int m = 0; m < NW; m++) {
+      int e = e0;
+      for (; e + 1 < e1; e += 2) {
+        R2 fa[NR], ua[G], fb[NR], ub[G];
+        int ma, mb;
+        ld(e, m, fa, ua, ma);
+        ld(e + 1, m, fb, ub, mb);
+        mac(m, fa, ua, ma);
+        mac(m, fb, ub, mb);
+      }
+      if (e < e1) {
+        R2 fa[NR], ua[G];
+        int ma;
+        ld(e, m, fa, ua, ma);
+        mac(m, fa, ua, ma);
+      }
+    }
+  } else
   for (int e = e0; e < e1; e++) {
     const R2* Fr = F + (size_t)gSrcF[e] * PP;
     const int mask = gMask[e];
@@ -845,10 +889,7 @@ __attribute__((amdgpu_waves_per_eu(m2lWavesPerEU<R, L, NR>(), 8))) void k_m2l(
 #pragma unroll
       for (int m = 0; m < NW; m++)
 #pragma unroll
-        for (int r = 0; r < NR; r++) {
-          acc[g][m][r].x += u[g][m].x * f[m][r].x - u[g][m].y * f[m][r].y;
-          acc[g][m][r].y += u[g][m].x * f[m][r].y + u[g][m].y * f[m][r].x;
-        }
+        for (int r = 0; r < NR; r++) cmacFma(acc[g][m][r], u[g][m], f[m][r]);
     }
   }
   const R inv = (R)(1.0 / (double)PP);
@@ -966,10 +1007,7 @@ __global__ __launch_bounds__(M2LShape<L>::TB) void k_m2l_col(
 #pragma unroll
     for (int m = 0; m < P; m++)
 #pragma unroll
-      for (int r = 0; r < NR; r++) {
-        acc[m][r].x += u[m].x * f[m][r].x - u[m].y * f[m][r].y;
-        acc[m][r].y += u[m].x * f[m][r].y + u[m].y * f[m][r].x;
-      }
+      for (int r = 0; r < NR; r++) cmacFma(acc[m][r], u[m], f[m][r]);
   }
   const int t = groupTarget[grp];
   if (t < 0) return;  // (uniform over the block)
@@ -1935,6 +1973,7 @@ struct Exec {
   int2 *p2mMfmaItems = nullptr, *p2mBigItems = nullptr;
   int np2mMfma = 0, np2mBig = 0;
   bool p2mMfma = true;       // float P2M in chunks of 16 right-hand sides on the matrix cores
+  bool m2lSliced = false;    // grouped M2L sweeps its entries once per frequency slice
   // (target leaf, first point) of the matrix-core L2P's point tiles, per
   // tile size (16 or 32 points: the order's l2pMfmaTile)
   int2* l2pTiles = nullptr;
@@ -2557,8 +2596,12 @@ static void launchM2L(Exec& X, hipStream_t s, int g0, int ng, const typename C2T
     hipLaunchKernelGGL((k_m2l_col<R, L, NR>), itemGrid(ng), dim3(M2LShape<L>::TB), 0, s,
                        X.gBeg + g0, X.gTarget + (size_t)g0 * G, X.gSrc, X.gMask, X.gCode, F,
                        X.fStride, (const R2*)X.U, Lb, X.lStride, oc.twc, oc.tws, ng);
+  else if (X.m2lSliced)
+    hipLaunchKernelGGL((k_m2l<R, L, NR, true>), itemGrid(ng), dim3(M2LShapeN<L, NR>::TB), 0, s,
+                       X.gBeg + g0, X.gTarget + (size_t)g0 * G, X.gSrc, X.gMask, X.gCode, F,
+                       X.fStride, (const R2*)X.U, Lb, X.lStride, oc.twc, oc.tws, ng);
   else
-    hipLaunchKernelGGL((k_m2l<R, L, NR>), itemGrid(ng), dim3(M2LShapeN<L, NR>::TB), 0, s,
+    hipLaunchKernelGGL((k_m2l<R, L, NR, false>), itemGrid(ng), dim3(M2LShapeN<L, NR>::TB), 0, s,
                        X.gBeg + g0, X.gTarget + (size_t)g0 * G, X.gSrc, X.gMask, X.gCode, F,
                        X.fStride, (const R2*)X.U, Lb, X.lStride, oc.twc, oc.tws, ng);
 }
@@ -3158,6 +3201,11 @@ int defmm_plan_set_option(defmm_plan plan, int option, int64_t value) {
                                     : option == DEFMM_OPT_L2P_MFMA ? X.l2pMfma : X.p2mMfma) =
           value != 0;
       return DEFMM_OK;
+    case DEFMM_OPT_M2L_SLICED:
+      if (value != 0 && value != 1) { g_err = "the option takes 0 or 1"; return DEFMM_EARG; }
+      if (X.quiesce() != dfm::OK) { g_err = "device synchronisation failed"; return DEFMM_EHIP; }
+      X.m2lSliced = value != 0;
+      return DEFMM_OK;
   }
   g_err = "unknown option";
   return DEFMM_EARG;
@@ -3212,7 +3260,20 @@ int defmm_plan_autotune(defmm_plan plan, int nrhs, int reps, char* report, int r
     std::snprintf(buf, sizeof buf, "kept m2l %s/%d\n", keep.mode == DEFMM_M2L_STORED ? "stored" : "batched", keep.rows);
     rep += buf;
   }
-  // 2. matrix cores per stage (mixed precision, chunks of 16 right-hand sides)
+  // 2. sliced sweep of the grouped M2L (orders <= 7)
+  if (X.L <= 7 && X.m2lMode != DEFMM_M2L_ON_THE_FLY) {
+    double t[2] = {0., 0.};
+    for (int on = 0; on < 2; on++) {
+      X.m2lSliced = on != 0;
+      int rc = fill();
+      if (rc || (rc = dfm::timePasses(X, nrhs, reps, t[on]))) return rc;
+    }
+    X.m2lSliced = t[1] < t[0];
+    std::snprintf(buf, sizeof buf, "m2l_sliced off %.3f ms, on %.3f ms: kept %s\n", t[0], t[1],
+                  X.m2lSliced ? "on" : "off");
+    rep += buf;
+  }
+  // 3. matrix cores per stage (mixed precision, chunks of 16 right-hand sides)
   if (X.prec == DEFMM_MIXED && nrhs >= dfm::kP2PMfmaNR) {
     struct Knob { const char* name; bool* v; };
     Knob knobs[3] = {{"p2p_mfma", &X.p2pMfma}, {"l2p_mfma", &X.l2pMfma}, {"p2m_mfma", &X.p2mMfma}};

@@ -40,6 +40,17 @@ __device__ __forceinline__ A cmulc(A a, B b) {
   return r;
 }
 
+// acc += u * f with four explicit fused multiply-adds in a fixed order, so
+// that every kernel forming the same sums rounds them the same way
+// (whatever contraction the compiler would choose in each context).
+template <typename A>
+__device__ __forceinline__ void cmacFma(A& acc, const A u, const A f) {
+  acc.x = fma(u.x, f.x, acc.x);
+  acc.x = fma(-u.y, f.y, acc.x);
+  acc.y = fma(u.x, f.y, acc.y);
+  acc.y = fma(u.y, f.x, acc.y);
+}
+
 // exp(i*arg).  double: sincosRed (sincos_red.hpp).  float: the argument is
 // reduced in double first (sincosRedF; beyond 2^20 rad modulo 2*pi), so that
 // a float evaluation keeps an absolute phase error near 1e-7 for arguments of

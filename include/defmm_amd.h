@@ -213,11 +213,16 @@ int defmm_plan_set_memory_limit(defmm_plan plan, int64_t bytes);
  * (1 by default: 16 right-hand sides per pass over a leaf's slots on the
  * matrix cores; 0: one right-hand side per pass on the vector units).
  * DEFMM_OPT_P2M_MFMA: the same for P2M (leaves of at most 64 points; larger
- * leaves stay on the vector units).
+ * leaves stay on the vector units).  DEFMM_OPT_M2L_SLICED (orders <= 7): 1
+ * sweeps each M2L work group's sources once per slice of its frequencies
+ * (a work group then reads a third of each row at a time, so rows shared
+ * with neighbouring work groups stay in L2 longer); 0 (default) sweeps them
+ * once.  Results are identical.
  */
 #define DEFMM_OPT_P2P_MFMA 1
 #define DEFMM_OPT_L2P_MFMA 2
 #define DEFMM_OPT_P2M_MFMA 3
+#define DEFMM_OPT_M2L_SLICED 4
 int defmm_plan_set_option(defmm_plan plan, int option, int64_t value);
 
 /*
@@ -229,7 +234,8 @@ int defmm_plan_set_option(defmm_plan plan, int option, int64_t value);
  *   1. M2L storage: stored Fourier rows (skipped when they do not fit) or
  *      the batched variant with a workspace of 1,024, 2,048, 4,096, 16,384
  *      or 65,536 rows;
- *   2. in mixed precision with nrhs >= 16: matrix cores on or off for P2P,
+ *   2. orders <= 7: the sliced M2L sweep on or off (DEFMM_OPT_M2L_SLICED);
+ *   3. in mixed precision with nrhs >= 16: matrix cores on or off for P2P,
  *      L2P and P2M (DEFMM_OPT_*_MFMA).
  * Writes a text report of every candidate's time and the kept values to
  * report (at most report_len bytes, NUL-terminated; report may be NULL).

@@ -296,3 +296,34 @@ def test_autotune_keeps_results():
     d = max(_diff(after[r], before[r])[0] for r in range(nr))
     _say(f"  after vs before: worst rel L2 {d:.3e}")
     assert d <= 1e-6
+
+
+@pytest.mark.parametrize("L,prec,nr", [(4, "mixed", 5), (5, "fp64", 3), (5, "mixed", 3),
+                                       (7, "fp64", 2), (7, "mixed", 3)])
+def test_m2l_sliced_sweep_identical(L, prec, nr):
+    """The grouped M2L kernel sweeping its entries once per frequency slice
+    (DEFMM_OPT_M2L_SLICED) equals the single sweep bit for bit: each
+    accumulator sums the same terms in the same order.  Several right-hand
+    sides (launches of NR = 1 .. 4) and the stored and batched variants."""
+    import defmm_amd as dm
+    from defmm_amd import workloads as wl
+    pts = wl.sphere(12000, seed=23)
+    kappa = wl.kappa_for(32, pts)
+    n = len(pts)
+    Q = np.stack([wl.charges(n, seed=1030 + r) for r in range(nr)])
+    p = dm.FP64 if prec == "fp64" else dm.MIXED
+    _say(f"case: sliced M2L sweep, sphere N={n} 32 wavelengths L={L}, {prec}, {nr} RHS")
+    A = _plan(dm, pts, L, 33, kappa, p)
+    A.set_rhs(nr)
+    for mode in (dm.M2L_STORED, dm.M2L_BATCHED):
+        A.set_m2l(mode, batch_rows=1024)
+        A.set_option(dm.OPT_M2L_SLICED, 0)
+        ref = A.apply_batch(Q)
+        A.set_option(dm.OPT_M2L_SLICED, 1)
+        out = A.apply_batch(Q)
+        same = bool(np.array_equal(out, ref))
+        _say(f"  m2l mode {mode}: identical {same}")
+        assert same
+    st = A.stats()
+    A.destroy()
+    assert st["hf_m2l"] > 0
