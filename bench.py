@@ -223,7 +223,7 @@ class Runner:
 
     def roofline(self, stage_ms, nrhs, mixed):
         """Roofline of the dominant kernel of a pass (M2L+F2L or P2P)."""
-        st, L = self.st, self.L
+        st, L = self.A.stats(), self.L  # (batch counts change with the M2L variant)
         dom = max(["m2l_f2l", "p2p"], key=lambda k: stage_ms[k])
         t = stage_ms[dom] * 1e-3
         mode = self.A.m2l_mode()
