@@ -2193,6 +2193,33 @@ static int buildBatches(Exec& X, std::vector<int>& local, std::vector<int>& rows
   return OK;
 }
 
+// Rows of the M2L table store from the kernel tables (set-up, on the GPU):
+// row k has key (t * 6 + s) * 8 + z: table t, axis order s, flips z;
+//   row[i] = U_t[(a_{s0} * P + a_{s1}) * P + a_{s2}],  i = (i0 * P + i1) * P + i2,
+// with a_j = P - i_j where bit j of z is set and i_j != 0, else i_j
+// (the index permutation of permutations.hpp:127-150 applied once per row).
+__constant__ int kTabAxisOrder[6][3] = {{0, 1, 2}, {1, 0, 2}, {1, 2, 0},
+                                        {0, 2, 1}, {2, 0, 1}, {2, 1, 0}};
+template <typename R2>
+__global__ void k_table_rows(const int* __restrict__ keys, int64_t nrows,
+                             const R2* __restrict__ tab, int P, R2* __restrict__ rows) {
+  const int64_t PP = (int64_t)P * P * P, n = nrows * PP;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += stride) {
+    const int64_t k = x / PP;
+    const int i = (int)(x - k * PP);
+    const int key = keys[k];
+    const int t = key / 48, s = (key / 8) % 6, z = key % 8;
+    int a[3] = {i / (P * P), (i / P) % P, i % P};
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+      if (((z >> j) & 1) && a[j]) a[j] = P - a[j];
+    const int src = (a[kTabAxisOrder[s][0]] * P + a[kTabAxisOrder[s][1]]) * P +
+                    a[kTabAxisOrder[s][2]];
+    rows[x] = tab[(int64_t)t * PP + src];
+  }
+}
+
 template <typename R>
 static int uploadTyped(Exec& X) {
   typedef typename C2T<R>::T R2;
@@ -2206,42 +2233,50 @@ static int uploadTyped(Exec& X) {
     a = toR2<R>(X.hSig2);
     CKR(X.upv(d, a));
     X.sig2 = d;
-    // one row per (table, axis order s, flips z) in use:
-    // row[w] = U_t[(a_{s0}, a_{s1}, a_{s2})] with a = flip_z(w)
-    static const int kVarSigma[6][3] = {{0, 1, 2}, {1, 0, 2}, {1, 2, 0},
-                                        {0, 2, 1}, {2, 0, 1}, {2, 1, 0}};
-    const int P = X.P, PP = X.PP;
+    // one row per (table, axis order, flips) in use, expanded on the GPU
+    // from the kernel tables (k_table_rows); a plan without M2L keeps one
+    // zero row that is never read
+    const int PP = X.PP;
     const size_t ntab = X.hTab.size() / (size_t)PP;
-    std::vector<R2> rowsU(X.hRowKey.size() * (size_t)PP, c2h<R>(0., 0.));
-    // (a plan without M2L keeps one zero row that is never read)
+    const size_t nrows = X.hRowKey.size();
     for (int key : X.hRowKey)
-      if (ntab > 0 && (size_t)(key / 48) >= ntab) {
+      if (ntab > 0 && (key < 0 || (size_t)(key / 48) >= ntab)) {
         g_err = "internal: M2L table id out of range";
         return ERR_STATE;
       }
-    auto fillRows = [&](size_t k0, size_t k1) {
-      for (size_t k = k0; k < k1 && ntab > 0; k++) {
-        const int key = X.hRowKey[k];
-        const int t = key / 48, v = (key / 8) % 6, z = key % 8;
-        for (int i = 0; i < PP; i++) {
-          int a[3] = {i / (P * P), (i / P) % P, i % P};
-          for (int j = 0; j < 3; j++)
-            if (((z >> j) & 1) && a[j]) a[j] = P - a[j];
-          const int src = (a[kVarSigma[v][0]] * P + a[kVarSigma[v][1]]) * P + a[kVarSigma[v][2]];
-          const cplx& c = X.hTab[(size_t)t * PP + src];
-          rowsU[k * PP + i] = c2h<R>(c.real(), c.imag());
+    void* rows = nullptr;
+    CKR(X.alloc(rows, nrows * (size_t)PP * sizeof(R2)));
+    X.U = rows;
+    if (ntab == 0) {
+      HIPCK(hipMemset(rows, 0, nrows * (size_t)PP * sizeof(R2)));
+    } else {
+      // the kernel tables and the keys are only needed here
+      std::vector<R2> tab = toR2<R>(X.hTab);
+      R2* dTab = nullptr;
+      int* dKeys = nullptr;
+      if (X.overBudget(tab.size() * sizeof(R2) + nrows * sizeof(int))) return ERR_MEMORY;
+      HIPCK(hipMalloc(&dTab, tab.size() * sizeof(R2)));
+      int rc = OK;
+      if (hipMalloc(&dKeys, nrows * sizeof(int)) != hipSuccess ||
+          hipMemcpy(dTab, tab.data(), tab.size() * sizeof(R2), hipMemcpyHostToDevice) !=
+              hipSuccess ||
+          hipMemcpy(dKeys, X.hRowKey.data(), nrows * sizeof(int), hipMemcpyHostToDevice) !=
+              hipSuccess) {
+        g_err = "M2L table upload failed";
+        rc = ERR_HIP;
+      } else {
+        const int64_t n = (int64_t)nrows * PP;
+        hipLaunchKernelGGL(k_table_rows<R2>, dim3(zeroGrid(n, 256)), dim3(256), 0, 0, dKeys,
+                           (int64_t)nrows, dTab, X.P, (R2*)rows);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+          g_err = "M2L table rows kernel failed";
+          rc = ERR_HIP;
         }
       }
-    };
-    {
-      const size_t nk = X.hRowKey.size();
-      const unsigned nth = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
-      std::vector<std::thread> th;
-      for (unsigned t = 0; t < nth; t++) th.emplace_back(fillRows, nk * t / nth, nk * (t + 1) / nth);
-      for (auto& x : th) x.join();
+      (void)hipFree(dKeys);
+      (void)hipFree(dTab);
+      if (rc != OK) return rc;
     }
-    CKR(X.upv(d, rowsU));
-    X.U = d;
   }
   {
     std::vector<R> c;
