@@ -449,11 +449,24 @@ int buildPlan(Plan& pl, const double* src, int64_t ns, const double* trg, int64_
   // thetis::kernels::Laplace, include/thetis/thetis_kernels.hpp:35): every
   // level is low-frequency and M2L starts at level 2, as in the reference's
   // initFrequency (include/tables/frequency.hpp:43-129)
-  if (!src || !trg || ns <= 0 || nt <= 0 || L < 2 || L > 12 || (ncrit == 0) ||
-      !(kappa >= 0) || ns > (int64_t)2000000000 || nt > (int64_t)2000000000) {
-    msg = "invalid argument";
+  if (!src || !trg) {
+    msg = "null point array";
     return ERR_ARG;
   }
+  if (ns <= 0 || nt <= 0 || ns > (int64_t)2000000000 || nt > (int64_t)2000000000) {
+    msg = "n_src and n_trg must be in 1 .. 2e9";
+    return ERR_ARG;
+  }
+  if (L < 2 || L > 12 || ncrit == 0 || !(kappa >= 0) || !std::isfinite(kappa)) {
+    msg = "invalid argument (order, Ncrit or kappa)";
+    return ERR_ARG;
+  }
+  for (const auto& [xyz, n] : {std::make_pair(src, ns), std::make_pair(trg, nt)})
+    for (int64_t i = 0; i < 3 * n; i++)
+      if (!std::isfinite(xyz[i])) {
+        msg = "non-finite point coordinate";
+        return ERR_ARG;
+      }
   pl = Plan();
   pl.L = L; pl.P = 2 * L - 1; pl.LL = L * L * L; pl.PP = pl.P * pl.P * pl.P;
   pl.ncrit = ncrit; pl.kappa = kappa;

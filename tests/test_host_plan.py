@@ -92,3 +92,29 @@ def test_fixed_depth_build():
     pts = np.random.default_rng(1).random((4000, 3))
     s = dm.build_plan_stats(pts, pts, 3, 0, 5.0, lvMAX=4)
     assert s["nlvl"] == 5
+
+
+def test_degenerate_inputs():
+    """Inputs the reference does not guard against give errors, not crashes:
+    no points, a single point (all points coincide), non-finite coordinates
+    or wavenumber, and a tree too shallow for any M2L level (two points)."""
+    dm, _ = _lib_or_skip()
+    rng = np.random.default_rng(3)
+    pts = rng.random((500, 3))
+    with pytest.raises(dm.DefmmError, match="n_src and n_trg"):
+        dm.build_plan_stats(np.zeros((0, 3)), pts, 4, 8, 1.0)
+    with pytest.raises(dm.DefmmError, match="n_src and n_trg"):
+        dm.build_plan_stats(pts, np.zeros((0, 3)), 4, 8, 1.0)
+    with pytest.raises(dm.DefmmError, match="coincide"):
+        dm.build_plan_stats(pts[:1], pts[:1], 4, 8, 1.0)
+    bad = pts.copy()
+    bad[17, 1] = np.nan
+    with pytest.raises(dm.DefmmError, match="non-finite"):
+        dm.build_plan_stats(bad, pts, 4, 8, 1.0)
+    bad[17, 1] = np.inf
+    with pytest.raises(dm.DefmmError, match="non-finite"):
+        dm.build_plan_stats(pts, bad, 4, 8, 1.0)
+    with pytest.raises(dm.DefmmError, match="invalid argument"):
+        dm.build_plan_stats(pts, pts, 4, 8, float("nan"))
+    with pytest.raises(dm.DefmmError, match="no M2L level"):
+        dm.build_plan_stats(pts[:2], pts[:2], 4, 8, 1.0)
