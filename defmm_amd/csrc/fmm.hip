@@ -2521,28 +2521,57 @@ static int upload(Exec& X, Plan& pl) {
   CKR(X.upv(X.gMask, pl.m2lGMask));
   {
     // (table id << 6 | h) -> row of the materialised table store << 3: one
-    // row per (table, axis order, flips) that some entry reads, in order of
-    // first use (codes of absent group targets point at row 0, never read)
+    // row per (table, axis order, flips) that some entry reads, in key order
+    // (codes of absent group targets point at row 0, never read).  Marking
+    // the keys in use and mapping the codes run in parallel over ranges of
+    // entries; the result does not depend on the thread count.
     const size_t nkeys = (size_t)std::max(1, pl.ntab) * 48;
+    const int G = pl.m2lG;
+    const size_t ne = pl.m2lGMask.size();
+    auto keyOf = [&](int c) {
+      return ((size_t)(c >> 6) * 6 + kVariantOfSigma[(c >> 3) & 7]) * 8 + (c & 7);
+    };
+    const unsigned nth = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+    auto parallel = [&](auto&& fn) {
+      std::vector<std::thread> th;
+      for (unsigned t = 0; t < nth; t++) th.emplace_back(fn, ne * t / nth, ne * (t + 1) / nth, t);
+      for (auto& x : th) x.join();
+    };
+    // per-thread marks of the keys in use (no shared writes), then merged
+    std::vector<std::vector<uint8_t>> used(nth);
+    std::vector<int> bad(nth, 0);
+    parallel([&](size_t e0, size_t e1, unsigned t) {
+      used[t].assign(nkeys, 0);
+      for (size_t e = e0; e < e1; e++)
+        for (int g = 0; g < G; g++)
+          if ((pl.m2lGMask[e] >> g) & 1) {
+            const size_t key = keyOf(pl.m2lGCode[e * G + g]);
+            if (key >= nkeys) bad[t] = 1;
+            else used[t][key] = 1;
+          }
+    });
+    for (int b : bad)
+      if (b) {
+        g_err = "internal: M2L table code out of range";
+        return ERR_STATE;
+      }
     std::vector<int> rowOf(nkeys, -1);
     X.hRowKey.clear();
-    std::vector<int> dc(pl.m2lGCode.size(), 0);
-    const int G = pl.m2lG;
-    for (size_t e = 0; e < pl.m2lGMask.size(); e++)
-      for (int g = 0; g < G; g++) {
-        if (!((pl.m2lGMask[e] >> g) & 1)) continue;
-        const int c = pl.m2lGCode[e * G + g];
-        const size_t key = ((size_t)(c >> 6) * 6 + kVariantOfSigma[(c >> 3) & 7]) * 8 + (c & 7);
-        if (key >= nkeys) {
-          g_err = "internal: M2L table code out of range";
-          return ERR_STATE;
-        }
-        if (rowOf[key] < 0) {
-          rowOf[key] = (int)X.hRowKey.size();
-          X.hRowKey.push_back((int)key);
-        }
-        dc[e * G + g] = rowOf[key] << 3;
+    for (size_t k = 0; k < nkeys; k++) {
+      bool u = false;
+      for (unsigned t = 0; t < nth && !u; t++) u = used[t][k] != 0;
+      if (u) {
+        rowOf[k] = (int)X.hRowKey.size();
+        X.hRowKey.push_back((int)k);
       }
+    }
+    std::vector<std::vector<uint8_t>>().swap(used);
+    std::vector<int> dc(pl.m2lGCode.size(), 0);
+    parallel([&](size_t e0, size_t e1, unsigned) {
+      for (size_t e = e0; e < e1; e++)
+        for (int g = 0; g < G; g++)
+          if ((pl.m2lGMask[e] >> g) & 1) dc[e * G + g] = rowOf[keyOf(pl.m2lGCode[e * G + g])] << 3;
+    });
     if (X.hRowKey.empty()) X.hRowKey.push_back(0);
     X.st.m2l_table_rows = (int64_t)X.hRowKey.size();
     CKR(X.upv(X.gCode, dc));
