@@ -977,16 +977,26 @@ int buildPlan(Plan& pl, const double* src, int64_t ns, const double* trg, int64_
   // cells read largely the same source F rows, so running them together
   // keeps those rows in cache.  Targets without M2L only need L = 0.
   {
-    std::vector<int> ord;
-    for (int s = 0; s < pl.T.nslots; s++) {
-      if (pl.m2lBeg[s + 1] > pl.m2lBeg[s]) ord.push_back(s);
-      else pl.m2lZero.push_back(s);
+    // counting sort by (level, direction), stable in slot order (an LF
+    // slot's direction -1 sorts first within its level)
+    const int nts = pl.T.nslots;
+    int maxDir = 1, maxLvl = 0;
+    for (int s = 0; s < nts; s++) {
+      maxDir = std::max(maxDir, pl.T.slotDir[s] + 2);
+      maxLvl = std::max(maxLvl, pl.T.lvl[pl.T.slotCell[s]]);
     }
-    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
-      int la = pl.T.lvl[pl.T.slotCell[a]], lb = pl.T.lvl[pl.T.slotCell[b]];
-      if (la != lb) return la < lb;
-      return pl.T.slotDir[a] < pl.T.slotDir[b];
-    });
+    auto keyOf = [&](int s) {
+      return (int64_t)pl.T.lvl[pl.T.slotCell[s]] * maxDir + (pl.T.slotDir[s] + 1);
+    };
+    const int64_t nk = (int64_t)(maxLvl + 1) * maxDir;
+    std::vector<int64_t> start(nk + 1, 0);
+    for (int s = 0; s < nts; s++)
+      if (pl.m2lBeg[s + 1] > pl.m2lBeg[s]) start[keyOf(s) + 1]++;
+      else pl.m2lZero.push_back(s);
+    for (int64_t k = 0; k < nk; k++) start[k + 1] += start[k];
+    std::vector<int> ord(start[nk]);
+    for (int s = 0; s < nts; s++)
+      if (pl.m2lBeg[s + 1] > pl.m2lBeg[s]) ord[start[keyOf(s)]++] = s;
     pl.m2lOrder.swap(ord);
   }
   // Groups of up to m2lGroupSize(L) consecutive targets of one level and direction
@@ -1065,22 +1075,40 @@ int buildPlan(Plan& pl, const double* src, int64_t ns, const double* trg, int64_
     int64_t total = 0;
     for (const Part& pt : parts) total += (int64_t)pt.src.size();
     if (total >= (int64_t)1 << 31) { msg = "M2L lists exceed 2^31 entries"; return ERR_MEMORY; }
-    pl.m2lGSrcF.reserve(total);
-    pl.m2lGMask.reserve(total);
-    pl.m2lGCode.reserve(total * G);
-    pl.m2lGroupBeg.reserve(ngroups + 1);
-    pl.m2lGroupTarget.reserve(ngroups * G);
-    pl.m2lGroupBeg.push_back(0);
+    pl.m2lGroupTarget.resize(ngroups * G);
+#pragma omp parallel for schedule(static)
     for (int64_t gi = 0; gi < ngroups; gi++) {
       const int ng = gStart[gi + 1] - gStart[gi];
-      for (int g = 0; g < G; g++) pl.m2lGroupTarget.push_back(g < ng ? pl.m2lOrder[gStart[gi] + g] : -1);
+      for (int g = 0; g < G; g++)
+        pl.m2lGroupTarget[gi * G + g] = g < ng ? pl.m2lOrder[gStart[gi] + g] : -1;
     }
-    for (Part& pt : parts) {
-      pl.m2lGSrcF.insert(pl.m2lGSrcF.end(), pt.src.begin(), pt.src.end());
-      pl.m2lGMask.insert(pl.m2lGMask.end(), pt.mask.begin(), pt.mask.end());
-      pl.m2lGCode.insert(pl.m2lGCode.end(), pt.code.begin(), pt.code.end());
-      for (int c : pt.nent) pl.m2lGroupBeg.push_back(pl.m2lGroupBeg.back() + c);
+    // the parts in range order: offsets first, then copied in parallel
+    std::vector<int64_t> eOff(nchunk + 1, 0), gOff(nchunk + 1, 0);
+    for (int64_t ch = 0; ch < nchunk; ch++) {
+      eOff[ch + 1] = eOff[ch] + (int64_t)parts[ch].src.size();
+      gOff[ch + 1] = gOff[ch] + (int64_t)parts[ch].nent.size();
+    }
+    pl.m2lGSrcF.resize(total);
+    pl.m2lGMask.resize(total);
+    pl.m2lGCode.resize(total * G);
+    pl.m2lGroupBeg.assign(ngroups + 1, 0);
+    {
+      // group starts: a running sum over the parts' entry counts
+      int64_t run = 0;
+      for (int64_t ch = 0; ch < nchunk; ch++)
+        for (size_t k = 0; k < parts[ch].nent.size(); k++) {
+          run += parts[ch].nent[k];
+          pl.m2lGroupBeg[gOff[ch] + (int64_t)k + 1] = (int)run;
+        }
+    }
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t ch = 0; ch < nchunk; ch++) {
+      Part& pt = parts[ch];
+      std::copy(pt.src.begin(), pt.src.end(), pl.m2lGSrcF.begin() + eOff[ch]);
+      std::copy(pt.mask.begin(), pt.mask.end(), pl.m2lGMask.begin() + eOff[ch]);
+      std::copy(pt.code.begin(), pt.code.end(), pl.m2lGCode.begin() + eOff[ch] * G);
       Part().src.swap(pt.src);
+      Part().mask.swap(pt.mask);
       Part().code.swap(pt.code);
     }
     pl.st.m2l_groups = ngroups;
