@@ -1004,6 +1004,9 @@ __global__ __launch_bounds__(M2LShape<L>::TB) void k_m2l_col(
       for (int r = 0; r < NR; r++) f[m][r] = Fr[r * fStride + m * P2];
 #pragma unroll
     for (int m = 0; m < P; m++) u[m] = Ur[m * P2];
+    // keep every load above this point: the scheduler would otherwise sink
+    // them next to their first use and keep only a few in flight per wave
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int m = 0; m < P; m++)
 #pragma unroll
