@@ -1491,18 +1491,31 @@ __global__ __launch_bounds__(256) void k_l2p_mfma(
       cre[b] = mfma_v4f{0.f, 0.f, 0.f, 0.f};
       cim[b] = mfma_v4f{0.f, 0.f, 0.f, 0.f};
     }
-    for (int ks = 0; ks < KS; ks++) {
-      const int kk = 4 * ks + j;
-      float2 bv = make_float2(0.f, 0.f);
-      if (kk < LL) {
-        const float2 lv2 = Ls[kk];
-        bv = hf ? cmul(lv2, sg[kk]) : lv2;
-      }
+    // KU steps' loads of L and of the phase row are issued together, ahead
+    // of their matrix-core steps (one load in flight per step would leave
+    // the loop bound by the load latency)
+    constexpr int KU = 8;
+    for (int ks0 = 0; ks0 < KS; ks0 += KU) {
+      float2 lv2[KU], sv[KU];
 #pragma unroll
-      for (int b = 0; b < NB; b++) {
-        const float a = S[16 * b + lr][kk];
-        cre[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bv.x, cre[b], 0, 0, 0);
-        cim[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bv.y, cim[b], 0, 0, 0);
+      for (int u = 0; u < KU; u++) {
+        const int kk = 4 * (ks0 + u) + j;
+        const bool on = ks0 + u < KS && kk < LL;
+        lv2[u] = on ? Ls[kk] : make_float2(0.f, 0.f);
+        sv[u] = (on && hf) ? sg[kk] : make_float2(1.f, 0.f);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int u = 0; u < KU; u++) {
+        if (ks0 + u >= KS) break;
+        const int kk = 4 * (ks0 + u) + j;
+        const float2 bv = hf ? cmul(lv2[u], sv[u]) : lv2[u];
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+          const float a = S[16 * b + lr][kk];
+          cre[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bv.x, cre[b], 0, 0, 0);
+          cim[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bv.y, cim[b], 0, 0, 0);
+        }
       }
     }
     // this lane holds C[point 16 b + 4 j + v][right-hand side lr]
