@@ -973,7 +973,9 @@ __global__ __launch_bounds__(M2LShape<L>::TB) void k_m2l_col(
   constexpr int P = 2 * L - 1, P2 = P * P, LL = L * L * L, PP = P * P * P;
   constexpr int TB = M2LShape<L>::TB;
   static_assert(M2LShape<L>::G == 1, "one target per group");
-  __shared__ R2 T1[L * P2], T2[L * L * P];
+  // order 8 runs each F2L step for all NR right-hand sides at once
+  constexpr int NT = P == 15 ? NR : 1;
+  __shared__ R2 T1[NT][L * P2], T2[NT][L * L * P];
   __shared__ R wc[P], ws[P];
   const int tid = threadIdx.x;
   const int b = itemIndex();
@@ -1015,6 +1017,44 @@ __global__ __launch_bounds__(M2LShape<L>::TB) void k_m2l_col(
   const int t = groupTarget[grp];
   if (t < 0) return;  // (uniform over the block)
   const R inv = (R)(1.0 / (double)PP);
+  if constexpr (P == 15) {
+    // prime-factor 15-point lines; each step for all right-hand sides, one
+    // barrier per step: axis 0 in registers, then a lane per line of axis 1
+    // (NR * L * P lines) and of axis 2 (NR * L^2 lines)
+    if (tid < P2)
+#pragma unroll
+      for (int rr = 0; rr < NR; rr++) {
+        R2 y[P];
+#pragma unroll
+        for (int m = 0; m < P; m++) y[m] = acc[m][rr];
+        dft15<1, P>(y);
+#pragma unroll
+        for (int k0 = 0; k0 < L; k0++) T1[rr][k0 * P2 + tid] = y[k0];
+      }
+    __syncthreads();
+    for (int x = tid; x < NR * L * P; x += TB) {
+      const int rr = x / (L * P), z = x - rr * (L * P);
+      const int k0 = z / P, v2 = z - k0 * P;
+      R2 y[P];
+#pragma unroll
+      for (int v1 = 0; v1 < P; v1++) y[v1] = T1[rr][(k0 * P + v1) * P + v2];
+      dft15<1, P>(y);
+#pragma unroll
+      for (int k1 = 0; k1 < L; k1++) T2[rr][(k0 * L + k1) * P + v2] = y[k1];
+    }
+    __syncthreads();
+    for (int x = tid; x < NR * L * L; x += TB) {
+      const int rr = x / (L * L), z = x - rr * (L * L);
+      R2 y[P];
+#pragma unroll
+      for (int v2 = 0; v2 < P; v2++) y[v2] = T2[rr][z * P + v2];
+      dft15<1, P>(y);
+      R2* Lr = Lout + rr * lStride + (size_t)t * LL + z * L;
+#pragma unroll
+      for (int k2 = 0; k2 < L; k2++) Lr[k2] = c2<R>(y[k2].x * inv, y[k2].y * inv);
+    }
+    return;
+  }
   __syncthreads();  // twiddles in LDS
 #pragma unroll
   for (int rr = 0; rr < NR; rr++) {
@@ -1022,52 +1062,20 @@ __global__ __launch_bounds__(M2LShape<L>::TB) void k_m2l_col(
     if (rr > 0) __syncthreads();  // T1 / T2 of the previous right-hand side consumed
     // axis 0 in registers: T1[k0 * P^2 + col] = sum_w0 J[w0, col] e^{+2 pi i w0 k0 / P}
     if (tid < P2) {
-      if constexpr (P == 15) {
-        R2 y[P];
 #pragma unroll
-        for (int m = 0; m < P; m++) y[m] = acc[m][rr];
-        dft15<1, P>(y);
+      for (int k0 = 0; k0 < L; k0++) {
+        R2 s = c2<R>(0, 0);
 #pragma unroll
-        for (int k0 = 0; k0 < L; k0++) T1[k0 * P2 + tid] = y[k0];
-      } else {
-#pragma unroll
-        for (int k0 = 0; k0 < L; k0++) {
-          R2 s = c2<R>(0, 0);
-#pragma unroll
-          for (int m = 0; m < P; m++) {
-            const int q = (m * k0) % P;
-            const R2 v = acc[m][rr];
-            s.x += v.x * wc[q] - v.y * ws[q];
-            s.y += v.y * wc[q] + v.x * ws[q];
-          }
-          T1[k0 * P2 + tid] = s;
+        for (int m = 0; m < P; m++) {
+          const int q = (m * k0) % P;
+          const R2 v = acc[m][rr];
+          s.x += v.x * wc[q] - v.y * ws[q];
+          s.y += v.y * wc[q] + v.x * ws[q];
         }
+        T1[0][k0 * P2 + tid] = s;
       }
     }
     __syncthreads();
-    if constexpr (P == 15) {
-      // axis 1, a lane per line (k0, w2): T2[(k0*L + k1)*P + w2], k1 < L
-      for (int x = tid; x < L * P; x += TB) {
-        const int k0 = x / P, v2 = x - k0 * P;
-        R2 y[P];
-#pragma unroll
-        for (int v1 = 0; v1 < P; v1++) y[v1] = T1[(k0 * P + v1) * P + v2];
-        dft15<1, P>(y);
-#pragma unroll
-        for (int k1 = 0; k1 < L; k1++) T2[(k0 * L + k1) * P + v2] = y[k1];
-      }
-      __syncthreads();
-      // axis 2 and scaling, a lane per line (k0, k1): L[(k0*L + k1)*L + k2]
-      for (int x = tid; x < L * L; x += TB) {
-        R2 y[P];
-#pragma unroll
-        for (int v2 = 0; v2 < P; v2++) y[v2] = T2[x * P + v2];
-        dft15<1, P>(y);
-#pragma unroll
-        for (int k2 = 0; k2 < L; k2++) Lr[x * L + k2] = c2<R>(y[k2].x * inv, y[k2].y * inv);
-      }
-      continue;
-    }
     // axis 1: T2[(k0*L + k1)*P + w2], k1 < L
     for (int x = tid; x < L * L * P; x += TB) {
       const int v2 = x % P, k1 = (x / P) % L, k0 = x / (L * P);
@@ -1075,11 +1083,11 @@ __global__ __launch_bounds__(M2LShape<L>::TB) void k_m2l_col(
 #pragma unroll
       for (int v1 = 0; v1 < P; v1++) {
         const int q = (v1 * k1) % P;
-        const R2 v = T1[(k0 * P + v1) * P + v2];
+        const R2 v = T1[0][(k0 * P + v1) * P + v2];
         s.x += v.x * wc[q] - v.y * ws[q];
         s.y += v.y * wc[q] + v.x * ws[q];
       }
-      T2[x] = s;
+      T2[0][x] = s;
     }
     __syncthreads();
     // axis 2 and scaling: L[(k0*L + k1)*L + k2]
@@ -1089,7 +1097,7 @@ __global__ __launch_bounds__(M2LShape<L>::TB) void k_m2l_col(
 #pragma unroll
       for (int v2 = 0; v2 < P; v2++) {
         const int q = (v2 * k2) % P;
-        const R2 v = T2[r * P + v2];
+        const R2 v = T2[0][r * P + v2];
         s.x += v.x * wc[q] - v.y * ws[q];
         s.y += v.y * wc[q] + v.x * ws[q];
       }
