@@ -206,21 +206,35 @@ def cmd_counters(a):
            "  cd /tmp && export TMPDIR=/tmp && cd /root/repo"]
     out += ["  " + c for c in a.cmd]
     out += [f"  python3 profile_summary.py counters <dir> --title '{a.title}'", "",
-            "Sums over every dispatch of the run.  GRBM_GUI_ACTIVE is reported summed over the "
+            "Sums over every dispatch of the run (TA_BUSY_avr: averaged over TA instances, then summed).  GRBM_GUI_ACTIVE is reported summed over the "
             f"{a.xcds} XCDs, so kernel cycles = GRBM_GUI_ACTIVE / {a.xcds}.  MFMA busy = "
             f"SQ_VALU_MFMA_BUSY_CYCLES / (cycles x {a.simds} SIMDs) (rocprofv3's MfmaUtil); VALU "
             f"busy = 4 x SQ_INSTS_VALU / (cycles x {a.simds} SIMDs) (a wave64 VALU instruction "
             "holds a 16-lane SIMD 4 cycles; an estimate: some instructions take longer).", ""]
+    hit = "TCC_HIT_sum" in names and "TCC_MISS_sum" in names
+    sq = "SQ_INSTS_VALU" in names
+    if hit:
+        out += ["L2 hit = TCC_HIT_sum / (TCC_HIT_sum + TCC_MISS_sum); TA busy = TA_BUSY_avr / kernel cycles.", ""]
     head = f"{'kernel':44s} {'calls':>6s} " + " ".join(f"{c:>24s}" for c in names)
-    out.append(head + "   MFMA busy  VALU busy")
+    ta = "TA_BUSY_avr" in names
+    out.append(head + ("   MFMA busy  VALU busy" if sq else "") + ("  L2 hit" if hit else "")
+               + ("  TA busy" if ta else ""))
     order = sorted(per, key=lambda k: -per[k].get("GRBM_GUI_ACTIVE", 0))
     for k in order[:a.top]:
         v = per[k]
         g = v.get("GRBM_GUI_ACTIVE", 0) / a.xcds * a.simds
         mb = 100 * v.get("SQ_VALU_MFMA_BUSY_CYCLES", 0) / g if g else 0
         vi = 400 * v.get("SQ_INSTS_VALU", 0) / g if g else 0
-        out.append(f"{k:44s} {len(calls[k]):6d} " + " ".join(f"{v.get(c, 0):24.0f}" for c in names)
-                   + f"   {mb:8.2f}%  {vi:9.2f}%")
+        row = f"{k:44s} {len(calls[k]):6d} " + " ".join(f"{v.get(c, 0):24.0f}" for c in names)
+        if sq:
+            row += f"   {mb:8.2f}%  {vi:9.2f}%"
+        if hit:
+            hm = v.get("TCC_HIT_sum", 0) + v.get("TCC_MISS_sum", 0)
+            row += f"  {100 * v.get('TCC_HIT_sum', 0) / hm:5.1f}%" if hm else "       -"
+        if ta:
+            cyc = v.get("GRBM_GUI_ACTIVE", 0) / a.xcds
+            row += f"  {100 * v.get('TA_BUSY_avr', 0) / cyc:6.1f}%" if cyc else "        -"
+        out.append(row)
     open(a.out, "w").write("\n".join(out) + "\n")
     print(f"wrote {a.out}")
 
