@@ -368,28 +368,40 @@ def batch_section(count, nmin, nmax, width, steps, warmup):
             "rel_l2_err_sampled": errs, "unit": "Mpts·RHS/s"}
 
 
-def multi_rhs_section(name, nrhs, width, steps, warmup, samples=200):
+def multi_rhs_section(name, nrhs, width, steps, warmup, samples=200, autotune=True):
     """A many-right-hand-side workload at full size: FP64 with one
     right-hand side, then mixed precision with nrhs right-hand sides evaluated
-    `width` at a time (M2L variant chosen by free memory); errors of the first
-    right-hand side on sampled targets against the FP64 direct sum."""
+    `width` at a time (each after the autotuner, one timed pass per
+    candidate); errors of the first right-hand side on sampled targets
+    against the FP64 direct sum."""
     import torch
     import defmm_amd as dm
     R = Runner(name, samples, steps, warmup)
+    tunes = {}
+    if autotune:
+        t0 = time.time()
+        tunes["fp64"] = R.A.autotune(1, reps=1)
+        log(f"autotune {name} fp64 ({time.time() - t0:.1f} s):\n{tunes['fp64']}")
     ms64, e64, _, st64 = R.timed(1, "fp64")
     R.A.set_precision(dm.MIXED)
     R.A.set_rhs(width)
+    if autotune:
+        t0 = time.time()
+        tunes["mixed_batched"] = R.A.autotune(width, reps=1)
+        log(f"autotune {name} mixed x{width} ({time.time() - t0:.1f} s):\n"
+            f"{tunes['mixed_batched']}")
     msb, eb, _, stb = R.timed(nrhs, f"mixed x{nrhs} (width {width})")
     out = {"workload": R.describe(), "setup_s": R.setup_s, "setup_ms_by_phase": R.setup_ms,
            "plan": {k: R.st[k] for k in ("p2p_pairs", "hf_m2l", "src_slots", "trg_slots",
                                          "m2f_slots", "m2l_table_rows")},
            "fp64": {"value": R.n / (ms64 * 1e-3) / 1e6, "ms_per_step": ms64, "rel_l2_err": e64,
-                    "stage_ms": st64},
+                    "stage_ms": st64, "autotune": tunes.get("fp64")},
            "mixed_batched": {"value": R.n * nrhs / (msb * 1e-3) / 1e6, "nrhs": nrhs,
                              "width": width, "ms_per_step": msb, "rel_l2_err_first_rhs": eb,
                              "stage_ms": stb, "speedup_vs_fp64": ms64 * nrhs / msb,
                              "m2l_variant": ["on the fly", "stored", "batched"][R.A.m2l_mode()],
-                             "device_gb": R.A.device_bytes() / 1e9},
+                             "device_gb": R.A.device_bytes() / 1e9,
+                             "autotune": tunes.get("mixed_batched")},
            "unit": "Mpts·RHS/s"}
     R.close()
     torch.cuda.empty_cache()
@@ -534,7 +546,7 @@ def main():
     many = None
     if args.multi_rhs_workload != "none":
         many = multi_rhs_section(args.multi_rhs_workload, args.multi_rhs, args.multi_rhs_width,
-                                 max(1, min(args.steps, 3)), 1)
+                                 max(1, min(args.steps, 3)), 1, autotune=not args.no_autotune)
 
     batch = None
     if args.batch_instances > 0:
