@@ -384,6 +384,9 @@ def multi_rhs_section(name, nrhs, width, steps, warmup, samples=200, autotune=Tr
         log(f"autotune {name} fp64 ({time.time() - t0:.1f} s):\n{tunes['fp64']}")
     ms64, e64, _, st64 = R.timed(1, "fp64")
     R.A.set_precision(dm.MIXED)
+    # (the FP64 tuning fixed the M2L variant for one right-hand side; the
+    # batched variant fits any width, and the tuning below chooses again)
+    R.A.set_m2l(dm.M2L_BATCHED)
     R.A.set_rhs(width)
     if autotune:
         t0 = time.time()

@@ -173,6 +173,7 @@ int defmm_direct(const double* src_xyz, int64_t n_src, const double* q, const do
   void *ds = nullptr, *dq = nullptr, *dt = nullptr, *dp = nullptr, *dw = nullptr;
   int rc = DEFMM_OK;
   auto ck = [&](hipError_t e, const char* what) {
+    if (e != hipSuccess) (void)hipGetLastError();  // consumed here, not at the next check
     if (e != hipSuccess && rc == DEFMM_OK) {
       g_err = std::string(what) + ": " + hipGetErrorString(e);
       rc = e == hipErrorOutOfMemory ? DEFMM_EMEMORY : DEFMM_EHIP;

@@ -1935,10 +1935,15 @@ __global__ __launch_bounds__(64) void k_p2p_reduce(const int* __restrict__ itemB
 // executor
 // ----------------------------------------------------------------------------
 
+// A failed call is reported here and its error consumed: HIP also keeps it
+// as the thread's last error, which the next launch check
+// (hipGetLastError) would otherwise report again, e.g. after an allocation
+// that did not fit and whose previous settings were restored.
 #define HIPCK(x)                                                              \
   do {                                                                        \
     hipError_t e_ = (x);                                                      \
     if (e_ != hipSuccess) {                                                   \
+      (void)hipGetLastError();                                                \
       g_err = std::string(#x) + ": " + hipGetErrorString(e_);                 \
       return e_ == hipErrorOutOfMemory ? ERR_MEMORY : ERR_HIP;                \
     }                                                                         \
@@ -2286,6 +2291,7 @@ static int uploadTyped(Exec& X) {
               hipSuccess ||
           hipMemcpy(dKeys, X.hRowKey.data(), nrows * sizeof(int), hipMemcpyHostToDevice) !=
               hipSuccess) {
+        (void)hipGetLastError();
         g_err = "M2L table upload failed";
         rc = ERR_HIP;
       } else {

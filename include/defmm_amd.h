@@ -237,6 +237,9 @@ int defmm_plan_set_option(defmm_plan plan, int option, int64_t value);
  *   2. orders <= 7: the sliced M2L sweep on or off (DEFMM_OPT_M2L_SLICED);
  *   3. in mixed precision with nrhs >= 16: matrix cores on or off for P2P,
  *      L2P and P2M (DEFMM_OPT_*_MFMA).
+ * The kept M2L variant is then the plan's explicit choice: a later width
+ * change (defmm_plan_set_rhs) keeps it, and fails with DEFMM_EMEMORY (the
+ * plan keeping its settings) if it does not fit at the new width.
  * Writes a text report of every candidate's time and the kept values to
  * report (at most report_len bytes, NUL-terminated; report may be NULL).
  * Waits for the plan's previous executions; synchronous.

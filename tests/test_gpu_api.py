@@ -327,3 +327,26 @@ def test_m2l_sliced_sweep_identical(L, prec, nr):
     st = A.stats()
     A.destroy()
     assert st["hf_m2l"] > 0
+
+
+def test_device_out_of_memory_keeps_plan():
+    """A reallocation that the device cannot satisfy (hipMalloc fails, as
+    opposed to the plan's own memory limit) returns DEFMM_EMEMORY and the plan
+    keeps its previous settings and results: the failed allocation's error is
+    not reported again by the restored upload or the next execution.  A 2M-point
+    ellipsoid at order 6 in mixed precision with stored Fourier rows takes
+    about 39 GB per right-hand side: 16 do not fit in one MI355X."""
+    import defmm_amd as dm
+    from defmm_amd import workloads as wl
+    pts, kappa, L, ncrit = wl.make("ellipsoid2m_256l_L6")
+    n = len(pts)
+    q = wl.charges(n, seed=1077)
+    _say(f"case: device out of memory, ellipsoid N={n} L={L} mixed stored, set_rhs(16)")
+    A = _plan(dm, pts, L, ncrit, kappa, dm.MIXED)
+    A.set_m2l(dm.M2L_STORED)
+    before = A.apply(q)
+    with pytest.raises(dm.DefmmError, match="out of memory.*keeps its previous settings"):
+        A.set_rhs(16)
+    after = A.apply(q)
+    A.destroy()
+    assert np.array_equal(after, before)
