@@ -2614,35 +2614,45 @@ static int upload(Exec& X, Plan& pl) {
     for (size_t k = 0; k < th.size(); k++) th[k] = (pl.m2lTabId[k] << 6) | pl.m2lHash[k];
     CKR(X.upv(X.m2lTabHash, th));
   }
-  // P2P work items: 64-target chunks x runs of <= kBudget sources
+  // P2P work items: 64-target chunks x runs of <= budget sources.  The
+  // budget is 4,096 sources unless that leaves fewer than 16,384 items (too
+  // few waves to fill 1,024 SIMDs: the 20,000-point cube has 3,249); then it
+  // is halved, down to 512.
   {
-    const int kBudget = 4096;
     std::vector<P2PItem> items;
     std::vector<int> itemBeg;
     const Tree& T = pl.T;
-    for (size_t lf = 0; lf < pl.p2pLeaf.size(); lf++) {
-      itemBeg.push_back((int)items.size());
-      int c = pl.p2pLeaf[lf];
-      for (int t0 = 0; t0 < T.nprt[c]; t0 += 64) {
-        int r = pl.p2pBeg[lf], off = 0, cnt = 0, r0 = r, off0 = 0;
-        for (; r < pl.p2pBeg[lf + 1]; r++) {
-          int len = pl.p2pSrcE[r] - pl.p2pSrcB[r];
-          off = 0;
-          while (off < len) {
-            int take = std::min(len - off, kBudget - cnt);
-            if (cnt == 0) { r0 = r; off0 = off; }
-            cnt += take;
-            off += take;
-            if (cnt == kBudget) {
-              items.push_back({(int)lf, t0, r0, off0, cnt});
-              cnt = 0;
+    auto build = [&](int kBudget) {
+      items.clear();
+      itemBeg.clear();
+      for (size_t lf = 0; lf < pl.p2pLeaf.size(); lf++) {
+        itemBeg.push_back((int)items.size());
+        int c = pl.p2pLeaf[lf];
+        for (int t0 = 0; t0 < T.nprt[c]; t0 += 64) {
+          int r = pl.p2pBeg[lf], off = 0, cnt = 0, r0 = r, off0 = 0;
+          for (; r < pl.p2pBeg[lf + 1]; r++) {
+            int len = pl.p2pSrcE[r] - pl.p2pSrcB[r];
+            off = 0;
+            while (off < len) {
+              int take = std::min(len - off, kBudget - cnt);
+              if (cnt == 0) { r0 = r; off0 = off; }
+              cnt += take;
+              off += take;
+              if (cnt == kBudget) {
+                items.push_back({(int)lf, t0, r0, off0, cnt});
+                cnt = 0;
+              }
             }
           }
+          if (cnt) items.push_back({(int)lf, t0, r0, off0, cnt});
         }
-        if (cnt) items.push_back({(int)lf, t0, r0, off0, cnt});
       }
+      itemBeg.push_back((int)items.size());
+    };
+    for (int budget = 4096;; budget /= 2) {
+      build(budget);
+      if (items.size() >= 16384 || budget <= 512) break;
     }
-    itemBeg.push_back((int)items.size());
     // the kernel walks ranges from (r, off) for cnt sources: they must stay
     // inside the item's leaf list
     for (const P2PItem& it : items) {
