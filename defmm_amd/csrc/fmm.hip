@@ -327,7 +327,8 @@ __global__ __launch_bounds__(128) void k_trans(
     const int* __restrict__ sgn, const int* __restrict__ sigIdx, const double2* __restrict__ edv,
     const typename C2T<R>::T* __restrict__ src, typename C2T<R>::T* __restrict__ dst,
     const typename C2T<R>::T* __restrict__ preTab, const typename C2T<R>::T* __restrict__ postTab,
-    const R* __restrict__ mat0, const R* __restrict__ mat1, int accumulate, int nItems) {
+    const R* __restrict__ mat0, const R* __restrict__ mat1, int accumulate, int nItems,
+    size_t rStride, int nr) {
   typedef typename C2T<R>::T R2;
   constexpr int LL = L * L * L;
   constexpr int NI = (LL + 127) / 128;
@@ -342,9 +343,13 @@ __global__ __launch_bounds__(128) void k_trans(
   // slots: give each XCD a contiguous part of the output order so that they
   // meet in one L2 (bijection of [0, n8), identity on the tail)
   const int blk = itemIndex();
-  if (blk >= nItems) return;
-  const int n8 = nItems & ~7;
-  const int o = blk < n8 ? (blk & 7) * (n8 >> 3) + (blk >> 3) : blk;
+  const int nAll = nItems * nr;  // (output, right-hand side) pairs
+  if (blk >= nAll) return;
+  const int n8 = nAll & ~7;
+  const int oa = blk < n8 ? (blk & 7) * (n8 >> 3) + (blk >> 3) : blk;
+  const int rr = oa / nItems, o = oa - rr * nItems;
+  src += rr * rStride;
+  dst += rr * rStride;
   const int e0 = beg[o], e1 = beg[o + 1];
   R2 acc[NI];
 #pragma unroll
@@ -466,7 +471,8 @@ __global__ __launch_bounds__(transLinesTB<L>()) void k_trans_lines(
     const int* __restrict__ sgn, const int* __restrict__ sigIdx, const double2* __restrict__ edv,
     const typename C2T<R>::T* __restrict__ src, typename C2T<R>::T* __restrict__ dst,
     const typename C2T<R>::T* __restrict__ preTab, const typename C2T<R>::T* __restrict__ postTab,
-    const R* __restrict__ mat0, const R* __restrict__ mat1, int accumulate, int nItems) {
+    const R* __restrict__ mat0, const R* __restrict__ mat1, int accumulate, int nItems,
+    size_t rStride, int nr) {
   typedef typename C2T<R>::T R2;
   constexpr int LL = L * L * L, NL = L * L, TB = transLinesTB<L>();
   constexpr int NI = (LL + TB - 1) / TB;
@@ -475,9 +481,13 @@ __global__ __launch_bounds__(transLinesTB<L>()) void k_trans_lines(
   const int lane = threadIdx.x;
   // XCD-contiguous output order, as in k_trans
   const int blk = itemIndex();
-  if (blk >= nItems) return;
-  const int n8 = nItems & ~7;
-  const int o = blk < n8 ? (blk & 7) * (n8 >> 3) + (blk >> 3) : blk;
+  const int nAll = nItems * nr;  // (output, right-hand side) pairs
+  if (blk >= nAll) return;
+  const int n8 = nAll & ~7;
+  const int oa = blk < n8 ? (blk & 7) * (n8 >> 3) + (blk >> 3) : blk;
+  const int rr = oa / nItems, o = oa - rr * nItems;
+  src += rr * rStride;
+  dst += rr * rStride;
   const int e0 = beg[o], e1 = beg[o + 1];
   // the line of this lane; lanes >= L^2 sweep line 0 and store nothing
   const bool own = lane < NL;
@@ -1959,17 +1969,28 @@ struct DevTrans {
 // k_trans_lines where it is the faster one (measured: float at every order,
 // double from L = 6; k_trans for double at L <= 5)
 template <typename R, int L>
+// One level's translations for nr right-hand sides in one launch (right-hand
+// side r of src and dst at r * rStride).
 static void launchTrans(const DevTrans& t, const typename C2T<R>::T* src,
                         typename C2T<R>::T* dst, const typename C2T<R>::T* pre,
                         const typename C2T<R>::T* post, const R* m0, const R* m1, int accumulate,
-                        hipStream_t s) {
-  if (sizeof(R) == 4 || L >= 6)
-    hipLaunchKernelGGL((k_trans_lines<R, L>), itemGrid(t.n), dim3(transLinesTB<L>()), 0, s,
-                       t.out, t.beg, t.in,
-                       t.sgn, t.sigIdx, t.edv, src, dst, pre, post, m0, m1, accumulate, t.n);
-  else
-    hipLaunchKernelGGL((k_trans<R, L>), itemGrid(t.n), dim3(128), 0, s, t.out, t.beg, t.in,
-                       t.sgn, t.sigIdx, t.edv, src, dst, pre, post, m0, m1, accumulate, t.n);
+                        size_t rStride, int nr, hipStream_t s) {
+  // (output, right-hand side) pairs are counted in int: chunks of right-hand sides
+  const int per = (int)std::max<int64_t>(1, std::min<int64_t>(nr, INT32_MAX / std::max(1, t.n)));
+  for (int r0 = 0; r0 < nr; r0 += per) {
+    const int c = std::min(per, nr - r0);
+    const int64_t n = (int64_t)t.n * c;
+    const typename C2T<R>::T* sr = src + r0 * rStride;
+    typename C2T<R>::T* dr = dst + r0 * rStride;
+    if (sizeof(R) == 4 || L >= 6)
+      hipLaunchKernelGGL((k_trans_lines<R, L>), itemGrid(n), dim3(transLinesTB<L>()), 0, s,
+                         t.out, t.beg, t.in, t.sgn, t.sigIdx, t.edv, sr, dr, pre, post, m0, m1,
+                         accumulate, t.n, rStride, c);
+    else
+      hipLaunchKernelGGL((k_trans<R, L>), itemGrid(n), dim3(128), 0, s, t.out, t.beg, t.in,
+                         t.sgn, t.sigIdx, t.edv, sr, dr, pre, post, m0, m1, accumulate, t.n,
+                         rStride, c);
+  }
 }
 
 struct Exec {
@@ -2838,9 +2859,9 @@ static int runTyped(Exec& X, const double2* qin, double2* pout, int parts, int n
     }
   mark(2);
   if (far)
-    for (int r = 0; r < nrhs; r++)
-      for (auto& t : X.m2m)
-        launchTrans<R, L>(t, (const R2*)Mr(r), Mr(r), sig, sig2, oc.A[0], oc.A[1], 0, s);
+    for (auto& t : X.m2m)
+      launchTrans<R, L>(t, (const R2*)Mr(0), Mr(0), sig, sig2, oc.A[0], oc.A[1], 0, X.mStride,
+                        nrhs, s);
   mark(3);
   const bool stored = X.m2lMode == DEFMM_M2L_STORED;
   if (far && X.nm2f && stored)
@@ -2892,9 +2913,9 @@ static int runTyped(Exec& X, const double2* qin, double2* pout, int parts, int n
       hipLaunchKernelGGL((k_zero_slots<R, L>), dim3(zeroGrid((int64_t)X.nM2LZero * X.LL, 256)),
                          dim3(256), 0, s, X.m2lZero, X.nM2LZero, Lr(r));
   if (far)
-    for (int r = 0; r < nrhs; r++)
-      for (auto& t : X.l2l)
-        launchTrans<R, L>(t, (const R2*)Lr(r), Lr(r), sig2, sig, oc.B[0], oc.B[1], 1, s);
+    for (auto& t : X.l2l)
+      launchTrans<R, L>(t, (const R2*)Lr(0), Lr(0), sig2, sig, oc.B[0], oc.B[1], 1, X.lStride,
+                        nrhs, s);
   mark(6);
   if (far && X.nl2p)
     for (int r = 0; r < nrhs;) {
