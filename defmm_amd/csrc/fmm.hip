@@ -603,17 +603,26 @@ constexpr int m2fRows() {
 // pass stores F straight from registers.  For P = 15 (order 8) a line is the
 // prime-factor 15-point DFT of dft.hpp (168 instead of 480 operations).
 template <typename R, int L>
+// Blocks for all nrhs right-hand sides of a launch: block b transforms rows
+// of right-hand side b / (blocks per right-hand side) (M + r * mStride into
+// F + r * fStride).
 __global__ __launch_bounds__(256) void k_m2f(const int* __restrict__ m2fSlot,
                                              const typename C2T<R>::T* __restrict__ M,
                                              typename C2T<R>::T* __restrict__ F,
                                              const R* __restrict__ twc,
-                                             const R* __restrict__ tws, int nRows) {
+                                             const R* __restrict__ tws, int nRows,
+                                             size_t mStride, size_t fStride, int nrhs) {
   typedef typename C2T<R>::T R2;
   constexpr int P = 2 * L - 1, LL = L * L * L, PP = P * P * P, NR = m2fRows<R, L>();
   __shared__ R2 T1[NR][L * L * P], T2[NR][L * P * P];
   const int tid = threadIdx.x;
-  const int row0 = itemIndex() * NR;
-  if (row0 >= nRows) return;
+  const int perRhs = (nRows + NR - 1) / NR;
+  const int bi = itemIndex();
+  if (bi >= perRhs * nrhs) return;
+  const int rhs = bi / perRhs;
+  M += rhs * mStride;
+  F += rhs * fStride;
+  const int row0 = (bi - rhs * perRhs) * NR;
   const int nr = min(NR, nRows - row0);
   R cw[P], sw[P];
 #pragma unroll
@@ -2864,11 +2873,18 @@ static int runTyped(Exec& X, const double2* qin, double2* pout, int parts, int n
                         nrhs, s);
   mark(3);
   const bool stored = X.m2lMode == DEFMM_M2L_STORED;
-  if (far && X.nm2f && stored)
-    for (int r = 0; r < nrhs; r++)
-      hipLaunchKernelGGL((k_m2f<R, L>), itemGrid((X.nm2f + m2fRows<R, L>() - 1) / m2fRows<R, L>()), dim3(256), 0, s,
-                         X.m2fSlot,
-                         (const R2*)Mr(r), Fr(r), oc.twc, oc.tws, X.nm2f);
+  // M2F blocks of nr right-hand sides in one launch (their count in int)
+  auto launchM2F = [&](const int* slots, int nRows, int r0, int nr, R2* F0) {
+    const int64_t perRhs = (nRows + m2fRows<R, L>() - 1) / m2fRows<R, L>();
+    const int per = (int)std::max<int64_t>(1, std::min<int64_t>(nr, INT32_MAX / std::max<int64_t>(1, perRhs)));
+    for (int a = 0; a < nr; a += per) {
+      const int c = std::min(per, nr - a);
+      hipLaunchKernelGGL((k_m2f<R, L>), itemGrid(perRhs * c), dim3(256), 0, s, slots,
+                         (const R2*)Mr(r0 + a), F0 + (size_t)a * X.fStride, oc.twc, oc.tws, nRows,
+                         X.mStride, X.fStride, c);
+    }
+  };
+  if (far && X.nm2f && stored) launchM2F(X.m2fSlot, X.nm2f, 0, nrhs, Fr(0));
   mark(4);
   if (far && X.nGroups) {
     if (stored) {
@@ -2898,10 +2914,7 @@ static int runTyped(Exec& X, const double2* qin, double2* pout, int parts, int n
         const int w0 = X.hBatchRowBeg[b], w1 = X.hBatchRowBeg[b + 1];
         for (int r0 = 0; r0 < nrhs;) {
           const int c = m2lChunkWidth<R, L>(nrhs - r0);
-          for (int r = r0; r < r0 + c; r++)
-            hipLaunchKernelGGL((k_m2f<R, L>), itemGrid((w1 - w0 + m2fRows<R, L>() - 1) / m2fRows<R, L>()), dim3(256),
-                               0, s, X.bRows + w0,
-                               (const R2*)Mr(r), Fr(r - r0), oc.twc, oc.tws, w1 - w0);
+          launchM2F(X.bRows + w0, w1 - w0, r0, c, Fr(0));
           r0 += m2lChunk<R, L>(X, s, g0, g1 - g0, r0, c, oc, Fr(0));
         }
       }
