@@ -3366,9 +3366,10 @@ int defmm_plan_autotune(defmm_plan plan, int nrhs, int reps, char* report, int r
   // 1. M2L storage variant
   {
     struct Cand { int mode, rows; };
-    std::vector<Cand> cands = {{DEFMM_M2L_STORED, X.batchRows}, {DEFMM_M2L_BATCHED, 1024},
-                               {DEFMM_M2L_BATCHED, 2048}, {DEFMM_M2L_BATCHED, 4096},
-                               {DEFMM_M2L_BATCHED, 16384}, {DEFMM_M2L_BATCHED, 65536}};
+    std::vector<Cand> cands = {{DEFMM_M2L_STORED, X.batchRows}, {DEFMM_M2L_BATCHED, 2048},
+                               {DEFMM_M2L_BATCHED, 4096},  {DEFMM_M2L_BATCHED, 8192},
+                               {DEFMM_M2L_BATCHED, 16384}, {DEFMM_M2L_BATCHED, 32768},
+                               {DEFMM_M2L_BATCHED, 65536}};
     const int mode0 = X.m2lMode, rows0 = X.batchRows;
     const bool auto0 = X.m2lAuto;
     double best = 1e300;

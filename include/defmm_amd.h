@@ -232,8 +232,8 @@ int defmm_plan_set_option(defmm_plan plan, int option, int64_t value);
  * per candidate (median), one knob at a time with the others fixed, and the
  * fastest value is kept:
  *   1. M2L storage: stored Fourier rows (skipped when they do not fit) or
- *      the batched variant with a workspace of 1,024, 2,048, 4,096, 16,384
- *      or 65,536 rows;
+ *      the batched variant with a workspace of 2,048 to 65,536 rows
+ *      (powers of two);
  *   2. orders <= 7: the sliced M2L sweep on or off (DEFMM_OPT_M2L_SLICED);
  *   3. in mixed precision with nrhs >= 16: matrix cores on or off for P2P,
  *      L2P and P2M (DEFMM_OPT_*_MFMA).
