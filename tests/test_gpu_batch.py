@@ -6,8 +6,9 @@ ellipsoid, 16 Gaussian clusters}, 4 - 32 wavelengths, orders 4 - 6, 1 - 4
 right-hand sides, mixed precision for most and FP64 for every fifth.
 
 Each instance's batched result must equal (bit for bit) its own evaluation
-by defmm_execute_batch_device, on two launches of the graph; three instances
-are compared with the oracle as well (FP64 to roundoff, mixed within 3e-6)."""
+by defmm_execute_batch_device, on three launches of the graph with an eager
+execution of one of the plans between them; three instances are compared
+with the oracle as well (FP64 to roundoff, mixed within 3e-6)."""
 import sys
 
 import numpy as np
@@ -77,15 +78,19 @@ def test_uneven_instance_batch_in_one_graph():
     torch.cuda.synchronize()
     B = dm.Batch(mats, [q.data_ptr() for q in Qs], [p.data_ptr() for p in Pbat],
                  [m[5] for m in meta], nbranch=8)
-    for launch in range(2):
+    scratch = torch.zeros_like(Pref[0])
+    for launch in range(3):
         for p in Pbat:
             p.fill_(9 + 9j)
+        # the plans' own eager work between the replays
+        mats[0].apply_batch_device(Qs[0].data_ptr(), scratch.data_ptr(), meta[0][5], s.cuda_stream)
         torch.cuda.synchronize()
         B.execute(s.cuda_stream)
         torch.cuda.synchronize()
         for m, a, b in zip(meta, Pref, Pbat):
             assert torch.equal(a, b), f"launch {launch}: instance {m[0]} differs"
-    _say("  every instance equals its own evaluation, on two launches")
+        assert torch.equal(scratch, Pref[0])
+    _say("  every instance equals its own evaluation, on three launches with eager passes between")
     # three instances against the oracle (FP64 and mixed)
     # the first FP64 instance and the first two mixed ones
     picks = [k for k, m in enumerate(meta) if m[6] == 0][:1] + \
