@@ -350,3 +350,29 @@ def test_device_out_of_memory_keeps_plan():
     after = A.apply(q)
     A.destroy()
     assert np.array_equal(after, before)
+
+
+def test_sixty_four_rhs_in_one_pass():
+    """The widest pass (64 right-hand sides, the largest width): four chunks
+    of 16 on the matrix cores and M2L launches of 2 (order 8) in float.  Every
+    fifth right-hand side against its own FP64 evaluation (within 3e-6), and
+    the pass equals the same right-hand sides evaluated 16 at a time."""
+    import defmm_amd as dm
+    from defmm_amd import workloads as wl
+    pts = wl.cube_edges(6000, seed=31)
+    kappa = wl.kappa_for(16, pts)
+    n, nr = len(pts), 64
+    Q = np.stack([wl.charges(n, seed=2000 + r) for r in range(nr)])
+    _say(f"case: 64 right-hand sides, edge-refined cube N={n} 16 wavelengths L=8, mixed")
+    A = _plan(dm, pts, 8, 65, kappa, dm.FP64)
+    ref = {r: A.apply(Q[r]) for r in range(0, nr, 5)}
+    A.set_precision(dm.MIXED)
+    A.set_rhs(nr)
+    P = A.apply_batch(Q)
+    A.set_rhs(16)
+    P16 = np.concatenate([A.apply_batch(Q[k:k + 16]) for k in range(0, nr, 16)])
+    A.destroy()
+    worst = max(_diff(P[r], ref[r])[0] for r in ref)
+    _say(f"  worst rel L2 vs FP64 over {len(ref)} right-hand sides {worst:.3e}")
+    assert worst <= 3e-6
+    assert np.array_equal(P, P16)
