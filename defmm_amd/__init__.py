@@ -21,7 +21,8 @@ import os
 import numpy as np
 
 __all__ = ["IBFMM_Mat", "Batch", "gemv", "DefmmError", "lib", "FP64", "MIXED", "STAT_NAMES",
-           "STAGE_NAMES", "build_plan_stats", "direct"]
+           "STAGE_NAMES", "build_plan_stats", "direct", "set_setup_mode", "SETUP_DEVICE",
+           "SETUP_HOST"]
 
 FP64 = 0
 MIXED = 1
@@ -29,6 +30,9 @@ MIXED = 1
 M2L_ON_THE_FLY = 0
 M2L_STORED = 1
 M2L_BATCHED = 2
+# where plan creation builds the dual walk and the M2L lists (defmm_amd.h: DEFMM_SETUP_*)
+SETUP_DEVICE = 0
+SETUP_HOST = 1
 # execution options (defmm_amd.h: DEFMM_OPT_*)
 OPT_P2P_MFMA = 1
 OPT_L2P_MFMA = 2
@@ -41,7 +45,7 @@ STAT_NAMES = [
     "lf_l2p", "hf_l2l", "hf2lf_l2l", "lf_l2l", "nlvl", "cpt", "max_stage", "ncells_t",
     "ncells_s", "p2p_items", "m2f_slots", "m2l_entries", "m2l_targets", "m2l_groups",
     "m2l_group_entries", "m2l_batches", "m2l_batch_rows", "ms_tree", "ms_walk", "ms_tables",
-    "ms_trans", "ms_m2l", "ms_p2p", "ms_upload", "m2l_table_rows",
+    "ms_trans", "ms_m2l", "ms_p2p", "ms_upload", "m2l_table_rows", "setup_device",
 ]
 STAGE_NAMES = ["gather", "p2m", "m2m", "m2f", "m2l_f2l", "l2l", "l2p", "p2p", "scatter"]
 
@@ -99,6 +103,7 @@ def lib():
     L.defmm_batch_execute.argtypes = [P, P]
     L.defmm_batch_destroy.argtypes = [P]
     L.defmm_direct.argtypes = [D, ctypes.c_int64, D, D, ctypes.c_int64, ctypes.c_double, D]
+    L.defmm_set_setup_mode.argtypes = [ctypes.c_int]
     _lib = L
     return L
 
@@ -116,6 +121,16 @@ def _dptr(a):
 def _points(a):
     a = np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(-1, 3))
     return a
+
+
+def set_setup_mode(mode):
+    """Where plan creation runs the dual walk and the M2L lists: SETUP_DEVICE
+    (the default, the current GPU) or SETUP_HOST (host C++).  Both build the
+    same plan.  Process-wide; returns the previous mode."""
+    rc = lib().defmm_set_setup_mode(int(mode))
+    if rc < 0:
+        _check(-rc)
+    return rc
 
 
 def build_plan_stats(src, trg, L, Ncrit, kappa, lvMAX=-1):

@@ -2,8 +2,9 @@
 
 The host plan builder (csrc/plan.cpp) is compiled by g++ without
 floating-point contraction (its discrete choices must not depend on the
-compiler); the device executor (csrc/fmm.hip) and the direct sum
-(csrc/direct.hip) by hipcc for gfx950 with packed FP32 instructions
+compiler); the device executor (csrc/fmm.hip), the direct sum (csrc/direct.hip) and
+the device set-up (csrc/setup.hip, also without contraction: it repeats the
+plan builder's decisions) by hipcc for gfx950 with packed FP32 instructions
 disabled.
 """
 import os
@@ -30,14 +31,16 @@ def build(verbose=True):
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-fopenmp",
           "-c", os.path.join(CSRC, "plan.cpp"), "-o", plan_o])
     direct_o = os.path.join(BUILD, "direct.o")
-    for src, obj in (("fmm.hip", fmm_o), ("direct.hip", direct_o)):
+    setup_o = os.path.join(BUILD, "setup.o")
+    for src, obj, contract in (("fmm.hip", fmm_o, "fast"), ("direct.hip", direct_o, "fast"),
+                               ("setup.hip", setup_o, "off")):
         _run([HIPCC, "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
               "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-              "-ffp-contract=fast", "-Wno-unused-result",
+              f"-ffp-contract={contract}", "-Wno-unused-result",
               "-c", os.path.join(CSRC, src), "-o", obj])
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", plan_o, fmm_o, direct_o, "-o", tmp,
-          "-lgomp", "-lpthread"])
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", plan_o, fmm_o, direct_o, setup_o,
+          "-o", tmp, "-lgomp", "-lpthread"])
     os.replace(tmp, LIB)
     return LIB
 

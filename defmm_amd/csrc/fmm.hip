@@ -6,10 +6,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <thread>
 #include <type_traits>
@@ -21,10 +23,12 @@
 #include "dft.hpp"
 #include "kernels.hpp"
 #include "plan.hpp"
+#include "xfer.hpp"
 
 namespace dfm {
 
 thread_local std::string g_err;  // declared in errors.hpp
+static std::atomic<int> g_setupMode{DEFMM_SETUP_DEVICE};  // defmm_set_setup_mode
 
 // Work-item grids (grid.hpp): lists of millions of items use a 2D grid; a
 // block's item index is blockIdx.y * gridDim.x + blockIdx.x.
@@ -2147,7 +2151,7 @@ struct Exec {
     bytes += (int64_t)(n * sizeof(X));
     if (typedPhase) typedBytes += (int64_t)(n * sizeof(X));
     dst = (X*)d;
-    if (src) HIPCK(hipMemcpy(d, src, n * sizeof(X), hipMemcpyHostToDevice));
+    if (src) HIPCK(copyLarge(d, src, n * sizeof(X), true));
     return OK;
   }
   template <class X>
@@ -3115,6 +3119,14 @@ const char* defmm_strerror(int code) {
 
 const char* defmm_last_error(void) { return g_err.c_str(); }
 
+int defmm_set_setup_mode(int mode) {
+  if (mode != DEFMM_SETUP_DEVICE && mode != DEFMM_SETUP_HOST) {
+    g_err = "unknown set-up mode";
+    return -DEFMM_EARG;
+  }
+  return dfm::g_setupMode.exchange(mode);
+}
+
 int defmm_plan_create(const double* src_xyz, int64_t n_src, const double* trg_xyz,
                       int64_t n_trg, int order, int ncrit, double kappa, int lvmax,
                       int precision, defmm_plan* out) {
@@ -3127,8 +3139,18 @@ int defmm_plan_create(const double* src_xyz, int64_t n_src, const double* trg_xy
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) { g_err = "no HIP device"; return DEFMM_ENODEVICE; }
   dfm::Plan* pl = new dfm::Plan();
   std::string msg;
-  int rc = dfm::buildPlan(*pl, src_xyz, n_src, trg_xyz, n_trg, order, lvmax >= 0 ? -lvmax : ncrit,
-                          kappa, msg);
+  int rc;
+  {
+    std::unique_ptr<dfm::SetupDevice> sd(
+        dfm::g_setupMode.load() == DEFMM_SETUP_DEVICE ? dfm::newSetupDevice() : nullptr);
+    rc = dfm::buildPlan(*pl, src_xyz, n_src, trg_xyz, n_trg, order, lvmax >= 0 ? -lvmax : ncrit,
+                        kappa, msg, sd.get());
+  }
+  if (rc == dfm::kSetupFallback) {
+    msg.clear();
+    rc = dfm::buildPlan(*pl, src_xyz, n_src, trg_xyz, n_trg, order, lvmax >= 0 ? -lvmax : ncrit,
+                        kappa, msg);
+  }
   if (rc != dfm::OK) {
     g_err = msg;
     delete pl;
@@ -3247,7 +3269,7 @@ int defmm_plan_stats(defmm_plan plan, int64_t* out, int n) {
                              s.m2l_entries, s.m2l_targets, s.m2l_groups,
                              s.m2l_group_entries, s.m2l_batches, s.m2l_batch_rows,
                              s.ms_tree, s.ms_walk, s.ms_tables, s.ms_trans, s.ms_m2l,
-                             s.ms_p2p, s.ms_upload, s.m2l_table_rows};
+                             s.ms_p2p, s.ms_upload, s.m2l_table_rows, s.setup_device};
   for (int i = 0; i < n && i < DEFMM_NSTATS; i++) out[i] = v[i];
   return DEFMM_OK;
 }

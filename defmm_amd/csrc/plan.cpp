@@ -229,6 +229,93 @@ void dft3(int P, const cplx* in, cplx* out, int sign) {
 }
 
 // ---------------------------------------------------------- the build ----
+// The offsets iZ = idx(target) - idx(source) of M2L pairs at a level are
+// bounded by the parents' non-admissible range: below the first
+// precomputation level by the parents' HF (or LF) admissibility reach, at
+// that level by the level's width.  Returns the half-width W of a window
+// |iZ_k| <= W that holds them, or -1 when the level has no M2L or the
+// window would exceed 2^26 offsets.
+int m2lWindow(const Plan& pl, int l) {
+  if (l < 0 || l >= pl.nlvl || !pl.precomp[l]) return -1;
+  int W = (1 << std::min(l, 20));
+  if (l > pl.maxStage) {
+    int pv = l - 1;
+    double tt = 2. * (pl.boxRad / double(1 << pv));
+    double reach = (pl.isHF[pv] && pv) ? std::max(tt * tt * pl.kappa, tt) / tt + 1. : 1.;
+    int bp = (int)std::ceil(reach * 1.01) + 1;
+    W = std::min(W, 2 * bp + 2);
+  }
+  size_t n = (size_t)(2 * W + 1) * (2 * W + 1) * (2 * W + 1);
+  return n > ((size_t)1 << 26) ? -1 : W;
+}
+
+// Table ids, permutation hashes and cones of the M2L offsets a device walk
+// met: rep[l] holds per window offset (index ((iZ0 + W) n + iZ1 + W) n +
+// iZ2 + W, n = 2W + 1) the smallest pair (target << 32 | source) with that
+// offset, or ~0.  The table keys (level, fundamental offset) are sorted and
+// numbered as the host walk numbers them; code[l] gets, per offset met,
+// table id | hash << 32 | (cone + 1) << 40 (cone -1 at LF levels), the cone
+// chosen from the representative pair as the host walk chooses it from the
+// first pair it meets with that offset (M2L.hpp:189-197).
+int offsetCodes(const Plan& pl, const std::vector<int>& W,
+                const std::vector<std::vector<uint64_t>>& rep,
+                std::vector<std::vector<uint64_t>>& code,
+                std::vector<std::array<int, 4>>& tabKeys, std::string& msg) {
+  tabKeys.clear();
+  code.assign(pl.nlvl, {});
+  auto offsetOf = [](int64_t k, int Wl, int iZ[3]) {
+    const int64_t n = 2 * Wl + 1;
+    iZ[2] = (int)(k % n) - Wl;
+    iZ[1] = (int)((k / n) % n) - Wl;
+    iZ[0] = (int)(k / (n * n)) - Wl;
+  };
+  for (int l = 0; l < pl.nlvl; l++) {
+    if (W[l] < 0 || rep[l].empty()) continue;
+    for (int64_t k = 0; k < (int64_t)rep[l].size(); k++) {
+      if (rep[l][k] == ~(uint64_t)0) continue;
+      int iZ[3], nI[3], h;
+      offsetOf(k, W[l], iZ);
+      fundamental(iZ, nI, h);
+      if (!(l && pl.isHF[l]))
+        for (int q = 0; q < 3; q++)
+          if (nI[q] >= pl.nCellPerDirMax[l]) {
+            msg = "M2L offset outside the table range";
+            return ERR_TABLE;
+          }
+      tabKeys.push_back({l, nI[0], nI[1], nI[2]});
+    }
+  }
+  std::sort(tabKeys.begin(), tabKeys.end());
+  tabKeys.erase(std::unique(tabKeys.begin(), tabKeys.end()), tabKeys.end());
+  for (int l = 0; l < pl.nlvl; l++) {
+    if (W[l] < 0 || rep[l].empty()) continue;
+    const bool hf = l && pl.isHF[l];
+    code[l].assign(rep[l].size(), 0);
+    // a cone search over all directions of the level per offset met
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t k = 0; k < (int64_t)rep[l].size(); k++) {
+      const uint64_t r = rep[l][k];
+      if (r == ~(uint64_t)0) continue;
+      int iZ[3], nI[3], h;
+      offsetOf(k, W[l], iZ);
+      fundamental(iZ, nI, h);
+      const std::array<int, 4> key = {l, nI[0], nI[1], nI[2]};
+      const int id = (int)(std::lower_bound(tabKeys.begin(), tabKeys.end(), key) - tabKeys.begin());
+      int cn = -1;
+      if (hf) {
+        const int i = (int)(r >> 32), j = (int)(r & 0xFFFFFFFFu);
+        double fZ[3];
+        for (int q = 0; q < 3; q++) fZ[q] = pl.T.ctr[3 * i + q] - pl.S.ctr[3 * j + q];
+        double nn = nrm3(fZ);
+        for (int q = 0; q < 3; q++) fZ[q] /= nn;
+        cn = bestCone(pl.dirs[pl.HFStage[l]], fZ);
+      }
+      code[l][k] = (uint64_t)(uint32_t)id | ((uint64_t)h << 32) | ((uint64_t)(cn + 1) << 40);
+    }
+  }
+  return OK;
+}
+
 namespace {
 struct PairLists {
   // M2L pairs in walk order: target cell, source cell, table id, hash, cone
@@ -256,24 +343,11 @@ struct Walker {
   std::vector<int> winW;
   int err = OK;
   Walker(Plan& p, PairLists* o) : pl(p), out(o) {
-    // the offsets of M2L pairs at a level are bounded by the parents'
-    // non-admissible range; a window of that half-width holds them all
     win.resize(pl.nlvl);
     winW.assign(pl.nlvl, -1);
     for (int l = 0; l < pl.nlvl; l++) {
-      if (!pl.precomp[l]) continue;
-      int W = (1 << std::min(l, 20));
-      if (l > pl.maxStage) {
-        int pv = l - 1;
-        double tt = 2. * (pl.boxRad / double(1 << pv));
-        double reach = (pl.isHF[pv] && pv) ? std::max(tt * tt * pl.kappa, tt) / tt + 1. : 1.;
-        int bp = (int)std::ceil(reach * 1.01) + 1;
-        W = std::min(W, 2 * bp + 2);
-      }
-      size_t n = (size_t)(2 * W + 1) * (2 * W + 1) * (2 * W + 1);
-      if (n > ((size_t)1 << 26)) continue;
-      winW[l] = W;
-      win[l].assign(n, 0);
+      winW[l] = m2lWindow(pl, l);
+      if (winW[l] >= 0) win[l].assign((size_t)(2 * winW[l] + 1) * (2 * winW[l] + 1) * (2 * winW[l] + 1), 0);
     }
   }
 
@@ -396,54 +470,80 @@ struct Walker {
 }  // namespace
 
 // dwnpass.hpp:80-104: a child needs direction d when its parent has any of
-// the refined directions 4d..4d+3.
+// the refined directions 4d..4d+3.  Level by level from the top, the cells of
+// a level in parallel (a child's marks are written by its parent only).
 static void propagate(Plan& pl, Tree& t) {
-  for (int c = 0; c < t.ncells(); c++) {
-    int l = t.lvl[c];
-    if (t.leaf(c) || !(pl.isHF[l] && pl.precomp[l])) continue;
-    if (!pl.isHF[l + 1]) continue;
-    for (int a = 0; a < t.nson[c]; a++) {
-      int s = t.sons[t.son0[c] + a];
-      for (int dt = 0; dt < pl.nDir[l + 1]; dt++)
-        for (int k = 0; k < 4; k++)
-          if (t.marks[t.mark0[c] + 4 * dt + k]) t.marks[t.mark0[s] + dt] = 1;
+  const int n = t.ncells();
+  std::vector<std::vector<int>> byLevel(pl.nlvl);
+  for (int c = 0; c < n; c++) byLevel[t.lvl[c]].push_back(c);
+  for (int l = 0; l + 1 < pl.nlvl; l++) {
+    if (!(pl.isHF[l] && pl.precomp[l]) || !pl.isHF[l + 1]) continue;
+    const std::vector<int>& cs = byLevel[l];
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t k = 0; k < (int64_t)cs.size(); k++) {
+      const int c = cs[k];
+      for (int a = 0; a < t.nson[c]; a++) {
+        int s = t.sons[t.son0[c] + a];
+        for (int dt = 0; dt < pl.nDir[l + 1]; dt++)
+          for (int q = 0; q < 4; q++)
+            if (t.marks[t.mark0[c] + 4 * dt + q]) t.marks[t.mark0[s] + dt] = 1;
+      }
     }
   }
 }
 
 // initdevs.hpp:187-245: HF cells get one slot per marked direction, in
 // ascending direction order; LF cells at precomputation levels one slot.
+// Slots are numbered in cell order: counts per cell, a prefix sum, then the
+// cells filled in parallel.
 static void numberSlots(Plan& pl, Tree& t) {
-  int n = t.ncells();
+  const int n = t.ncells();
   t.slot0.assign(n, 0);
   t.nslot.assign(n, 0);
-  t.slotCell.clear();
-  t.slotDir.clear();
+#pragma omp parallel for schedule(dynamic, 256)
   for (int c = 0; c < n; c++) {
-    int l = t.lvl[c];
-    t.slot0[c] = (int)t.slotCell.size();
+    const int l = t.lvl[c];
     if (!pl.precomp[l]) continue;
     if (pl.isHF[l]) {
-      for (int d = 0; d < pl.nDir[l]; d++)
-        if (t.marks[t.mark0[c] + d]) { t.slotCell.push_back(c); t.slotDir.push_back(d); }
+      int k = 0;
+      for (int d = 0; d < pl.nDir[l]; d++) k += t.marks[t.mark0[c] + d] ? 1 : 0;
+      t.nslot[c] = k;
     } else {
-      t.slotCell.push_back(c);
-      t.slotDir.push_back(-1);
+      t.nslot[c] = 1;
     }
-    t.nslot[c] = (int)t.slotCell.size() - t.slot0[c];
   }
-  t.nslots = (int)t.slotCell.size();
+  int64_t run = 0;
+  for (int c = 0; c < n; c++) {
+    t.slot0[c] = (int)run;
+    run += t.nslot[c];
+  }
+  t.nslots = (int)run;
+  t.slotCell.assign(run, 0);
+  t.slotDir.assign(run, -1);
   t.slotIdx.assign(t.marks.size(), -1);
-  for (int s = 0; s < t.nslots; s++) {
-    int c = t.slotCell[s];
-    if (t.slotDir[s] >= 0) t.slotIdx[t.mark0[c] + t.slotDir[s]] = s;
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int c = 0; c < n; c++) {
+    const int l = t.lvl[c];
+    if (!pl.precomp[l]) continue;
+    int s = t.slot0[c];
+    if (pl.isHF[l]) {
+      for (int d = 0; d < pl.nDir[l]; d++)
+        if (t.marks[t.mark0[c] + d]) {
+          t.slotCell[s] = c;
+          t.slotDir[s] = d;
+          t.slotIdx[t.mark0[c] + d] = s;
+          s++;
+        }
+    } else {
+      t.slotCell[s] = c;
+    }
   }
 }
 
 int validatePlan(const Plan& pl, std::string& msg);
 
 int buildPlan(Plan& pl, const double* src, int64_t ns, const double* trg, int64_t nt, int L,
-              int ncrit, double kappa, std::string& msg) {
+              int ncrit, double kappa, std::string& msg, SetupDevice* dev) {
   const int lvmax = ncrit < 0 ? -ncrit : -1;  // negative Ncrit: fixed depth
   // kappa = 0 is the Laplace kernel 1/(4 pi r) (the reference's
   // thetis::kernels::Laplace, include/thetis/thetis_kernels.hpp:35): every
@@ -547,7 +647,13 @@ int buildPlan(Plan& pl, const double* src, int64_t ns, const double* trg, int64_
   // dual walk
   PairLists pr;
   std::vector<std::array<int, 4>> tabKeys;  // (level, fundamental offset) by table id
-  {
+  if (dev) {
+    // on the device: marks, table keys, statistics and P2P pairs here; the
+    // M2L pairs stay on the device for its M2L lists
+    int r = dev->walk(pl, tabKeys, pr.pt, pr.ps, msg);
+    if (r != OK) return r;
+    pl.st.setup_device = 1;
+  } else {
     // The walk from the roots is split at a level with enough cell pairs:
     // a serial walk records the pairs reached at that level as tasks (and
     // lists the pairs decided above it); the tasks are walked in parallel,
@@ -905,71 +1011,76 @@ int buildPlan(Plan& pl, const double* src, int64_t ns, const double* trg, int64_
 
   pl.st.ms_trans = tick();
   // ---- M2L, CSR by target slot; M2F only for slots read by M2L ----
-  int64_t nm = (int64_t)pr.mt.size();
-  std::vector<int> tslot(nm), sslot(nm);
-  int missing = 0;
+  if (dev) {
+    int r = dev->m2lLists(pl, msg);
+    if (r != OK) return r;
+  } else {
+    int64_t nm = (int64_t)pr.mt.size();
+    std::vector<int> tslot(nm), sslot(nm);
+    int missing = 0;
 #pragma omp parallel for schedule(static) reduction(+ : missing)
-  for (int64_t e2 = 0; e2 < nm; e2++) {
-    int c = pr.mcone[e2];
-    tslot[e2] = pl.T.slotOf(pr.mt[e2], c);
-    sslot[e2] = pl.S.slotOf(pr.ms[e2], c);
-    if (tslot[e2] < 0 || sslot[e2] < 0) missing++;
-  }
-  if (missing) { msg = "internal: missing M2L slot"; return ERR_STATE; }
-  {
-    std::vector<uint8_t> used(pl.S.nslots, 0);
-#pragma omp parallel for schedule(static)
-    for (int64_t e2 = 0; e2 < nm; e2++) __atomic_store_n(&used[sslot[e2]], (uint8_t)1, __ATOMIC_RELAXED);
-    pl.m2fIndex.assign(pl.S.nslots, -1);
-    for (int s = 0; s < pl.S.nslots; s++)
-      if (used[s]) { pl.m2fIndex[s] = (int)pl.m2fSlot.size(); pl.m2fSlot.push_back(s); }
-  }
-  // CSR by target slot, stable (entries of a target in walk order): a
-  // counting sort over contiguous chunks of the entries, one chunk per thread
-  pl.m2lBeg.assign(pl.T.nslots + 1, 0);
-  pl.m2lSrcF.resize(nm); pl.m2lTabId.resize(nm); pl.m2lHash.resize(nm);
-  {
-    int nth = 1;
-#ifdef _OPENMP
-    nth = omp_get_max_threads();
-#endif
-    const int nts = pl.T.nslots;
-    std::vector<std::vector<int>> cnt(nth);
-#pragma omp parallel num_threads(nth)
+    for (int64_t e2 = 0; e2 < nm; e2++) {
+      int c = pr.mcone[e2];
+      tslot[e2] = pl.T.slotOf(pr.mt[e2], c);
+      sslot[e2] = pl.S.slotOf(pr.ms[e2], c);
+      if (tslot[e2] < 0 || sslot[e2] < 0) missing++;
+    }
+    if (missing) { msg = "internal: missing M2L slot"; return ERR_STATE; }
     {
-      int t = 0;
+      std::vector<uint8_t> used(pl.S.nslots, 0);
+#pragma omp parallel for schedule(static)
+      for (int64_t e2 = 0; e2 < nm; e2++) __atomic_store_n(&used[sslot[e2]], (uint8_t)1, __ATOMIC_RELAXED);
+      pl.m2fIndex.assign(pl.S.nslots, -1);
+      for (int s = 0; s < pl.S.nslots; s++)
+        if (used[s]) { pl.m2fIndex[s] = (int)pl.m2fSlot.size(); pl.m2fSlot.push_back(s); }
+    }
+    // CSR by target slot, stable (entries of a target in walk order): a
+    // counting sort over contiguous chunks of the entries, one chunk per thread
+    pl.m2lBeg.assign(pl.T.nslots + 1, 0);
+    pl.m2lSrcF.resize(nm); pl.m2lTabId.resize(nm); pl.m2lHash.resize(nm);
+    {
+      int nth = 1;
 #ifdef _OPENMP
-      t = omp_get_thread_num();
+      nth = omp_get_max_threads();
 #endif
-      const int64_t a = nm * t / nth, b = nm * (t + 1) / nth;
-      std::vector<int>& c = cnt[t];
-      c.assign(nts, 0);
-      for (int64_t e2 = a; e2 < b; e2++) c[tslot[e2]]++;
+      const int nts = pl.T.nslots;
+      std::vector<std::vector<int>> cnt(nth);
+#pragma omp parallel num_threads(nth)
+      {
+        int t = 0;
+#ifdef _OPENMP
+        t = omp_get_thread_num();
+#endif
+        const int64_t a = nm * t / nth, b = nm * (t + 1) / nth;
+        std::vector<int>& c = cnt[t];
+        c.assign(nts, 0);
+        for (int64_t e2 = a; e2 < b; e2++) c[tslot[e2]]++;
 #pragma omp barrier
 #pragma omp for schedule(static)
-      for (int s = 0; s < nts; s++) {
-        int run = 0;
-        for (int u = 0; u < nth; u++) {
-          int v = cnt[u][s];
-          cnt[u][s] = run;  // offset of chunk u within slot s
-          run += v;
+        for (int s = 0; s < nts; s++) {
+          int run = 0;
+          for (int u = 0; u < nth; u++) {
+            int v = cnt[u][s];
+            cnt[u][s] = run;  // offset of chunk u within slot s
+            run += v;
+          }
+          pl.m2lBeg[s + 1] = run;
         }
-        pl.m2lBeg[s + 1] = run;
-      }
 #pragma omp single
-      for (int s = 0; s < nts; s++) pl.m2lBeg[s + 1] += pl.m2lBeg[s];
-      // implicit barrier after single
-      for (int64_t e2 = a; e2 < b; e2++) {
-        const int s = tslot[e2];
-        const int64_t pos = (int64_t)pl.m2lBeg[s] + c[s]++;
-        pl.m2lSrcF[pos] = pl.m2fIndex[sslot[e2]];
-        pl.m2lTabId[pos] = pr.mtab[e2];
-        pl.m2lHash[pos] = pr.mh[e2];
+        for (int s = 0; s < nts; s++) pl.m2lBeg[s + 1] += pl.m2lBeg[s];
+        // implicit barrier after single
+        for (int64_t e2 = a; e2 < b; e2++) {
+          const int s = tslot[e2];
+          const int64_t pos = (int64_t)pl.m2lBeg[s] + c[s]++;
+          pl.m2lSrcF[pos] = pl.m2fIndex[sslot[e2]];
+          pl.m2lTabId[pos] = pr.mtab[e2];
+          pl.m2lHash[pos] = pr.mh[e2];
+        }
       }
     }
   }
   pl.st.m2f_slots = (int64_t)pl.m2fSlot.size();
-  pl.st.m2l_entries = (int64_t)nm;
+  pl.st.m2l_entries = (int64_t)pl.m2lSrcF.size();
   for (int s = 0; s < pl.T.nslots; s++)
     if (pl.m2lBeg[s + 1] > pl.m2lBeg[s]) pl.st.m2l_targets++;
   // Processing order of the M2L targets: by level, then direction, then

@@ -78,6 +78,7 @@ struct Stats {
   int64_t ms_tree = 0, ms_walk = 0, ms_tables = 0, ms_trans = 0, ms_m2l = 0, ms_p2p = 0;
   int64_t ms_upload = 0;  // device upload (set by the executor)
   int64_t m2l_table_rows = 0;  // rows of the materialised M2L table store (set at upload)
+  int64_t setup_device = 0;    // 1: the dual walk and M2L lists were built on the GPU
 };
 
 // A per-level translation list (M2M or L2L): for every output slot, the
@@ -148,9 +149,41 @@ struct Plan {
   Stats st;
 };
 
+// Set-up steps that a device can take over from the host builder.  walk()
+// runs where the host dual walk would: it sets the trees' effective-direction
+// marks and the walk statistics, returns the sorted table keys (level,
+// fundamental offset) and the P2P cell pairs, and keeps the M2L pairs.
+// m2lLists() runs where the host builds the M2L lists: it fills m2fIndex,
+// m2fSlot and the CSR m2lBeg / m2lSrcF / m2lTabId / m2lHash, with the
+// entries of a target slot in ascending source cell order (the host walk's
+// order).  Either returns OK, an error code, or kSetupFallback when the
+// device cannot take this plan (the caller then builds it on the host).
+constexpr int kSetupFallback = -1;
+struct SetupDevice {
+  virtual ~SetupDevice() {}
+  virtual int walk(Plan& pl, std::vector<std::array<int, 4>>& tabKeys, std::vector<int>& pt,
+                   std::vector<int>& ps, std::string& msg) = 0;
+  virtual int m2lLists(Plan& pl, std::string& msg) = 0;
+};
+
 // Build the plan.  Returns an error code (OK on success); msg gets details.
+// With dev, the dual walk and the M2L lists are built by it.
 int buildPlan(Plan& pl, const double* src, int64_t ns, const double* trg, int64_t nt, int L,
-              int ncrit, double kappa, std::string& msg);
+              int ncrit, double kappa, std::string& msg, SetupDevice* dev = nullptr);
+
+// Half-width W of the offset window |iZ_k| <= W that holds every M2L pair of
+// level l, or -1 (no M2L at l, or more than 2^26 offsets).
+int m2lWindow(const Plan& pl, int l);
+
+// Table keys and per-offset codes (table id | hash << 32 | (cone + 1) << 40)
+// from the representative pair of each offset met at each level.
+int offsetCodes(const Plan& pl, const std::vector<int>& W,
+                const std::vector<std::vector<uint64_t>>& rep,
+                std::vector<std::vector<uint64_t>>& code,
+                std::vector<std::array<int, 4>>& tabKeys, std::string& msg);
+
+// The device set-up (setup.hip): nullptr when no device is usable.
+SetupDevice* newSetupDevice();
 
 // Separable unnormalised DFT of a P^3 array (sign -1: forward).
 void dft3(int P, const cplx* in, cplx* out, int sign);

@@ -1,0 +1,779 @@
+// setup.hip -- the plan builder's dual walk and M2L lists on the GPU.
+//
+// The host builder (plan.cpp) walks the source and target octrees from the
+// roots (hrzpass.hpp:61-91 and the set-up walk hrzpass.hpp:111-138 of the
+// reference): a pair of same-level cells is an M2L pair when it passes the
+// level's admissibility test (hrzpass.hpp:39-56), a P2P pair when it does not
+// and either cell is a leaf, and otherwise the pairs of their children are
+// visited.  Here the walk runs level by level: the sources that target cell
+// i meets at level l are the children of the sources its parent expanded at
+// level l - 1.  One wavefront handles one target cell; it takes its
+// parent's expanded sources in order, eight per step with one lane per
+// child, classifies the 64 candidate pairs and appends each class in lane
+// order (ballot + prefix count).  A count pass sizes every target's segment
+// first, so each target's M2L, P2P and expansion lists come out in
+// ascending source cell order without a sort: within one target the host's
+// depth-first walk meets its sources in that same order (sources that first
+// differ at some level are visited in the order of their child index there,
+// which is pre-order).
+//
+// The M2L pairs stay on the device.  Each offset iZ = idx(target) -
+// idx(source) a level meets gets a representative pair (the smallest
+// (target, source), by a 64-bit atomic minimum over a dense window of
+// offsets); the host turns the representatives into table ids, permutation
+// hashes and cones (offsetCodes, plan.cpp), and the device then marks the
+// effective directions of both cells of every HF pair.  After the host has
+// numbered the slots, the device builds the M2L CSR by target slot: a count
+// per slot (atomics), the host's prefix sum, then a stable placement in
+// which the wavefront of a target cell places its pairs in segment order,
+// one distinct target slot at a time (a target slot belongs to one cell,
+// so its cursor has one writer).
+//
+// Every decision is computed as on the host: the HF test in double
+// precision with the same operations in the same order and no contraction
+// (this file is compiled with -ffp-contract=off), the LF test and the
+// offsets in integers.  The resulting plan is the host builder's, entry for
+// entry.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "plan.hpp"
+#include "xfer.hpp"
+
+namespace dfm {
+namespace {
+
+constexpr int kWaves = 4;  // wavefronts (target cells) per 256-thread block
+
+struct DTree {
+  const int* idx;      // 3 per cell
+  const double* ctr;   // 3 per cell
+  const double* rad;
+  const int* son0;
+  const int* nson;
+  const int* sons;
+};
+
+struct WalkArgs {
+  DTree T, S;
+  int nt;              // target cells at this level
+  const int* tcell;    // their cell ids
+  const int* tpar;     // their parents' positions in the previous level's list
+  const int* eBeg;     // previous level: sources expanded per target (CSR)
+  const int* eSrc;
+  int kind;            // 0: no M2L at this level, 1: HF test, 2: LF test
+  double kappa;
+  int* cM;             // count pass: per target M2L, P2P and expansion counts
+  int* cP;
+  int* cE;
+  const int* oM;       // write pass: per target offsets (exclusive sums)
+  const int* oP;
+  const int* oE;
+  int* mSrc;
+  int* pSrc;
+  int* eOut;
+  unsigned long long* rep;  // representative pair per offset of the window
+  int W;
+  int* overflow;       // set when an M2L offset falls outside the window
+};
+
+// hrzpass.hpp:51-56, the operations of the host's macHF in the same order
+__device__ __forceinline__ bool macHF(double t0, double t1, double t2, const double* sc,
+                                      double sr, double kappa) {
+#pragma clang fp contract(off)
+  const double d0 = t0 - sc[0], d1 = t1 - sc[1], d2 = t2 - sc[2];
+  double s = 0.;
+  s += d0 * d0;
+  s += d1 * d1;
+  s += d2 * d2;
+  const double tmp = (1. - 1.e-8) * (sqrt(s) - 2. * sr);
+  const double tt = 2. * sr;
+  const double a = tt * tt * kappa;
+  return tmp >= (a < tt ? tt : a);
+}
+
+// hrzpass.hpp:39-49
+__device__ __forceinline__ bool macLF(int z0, int z1, int z2) {
+  const int test = abs(z0) + abs(z1) + abs(z2);
+  const int ncmp = (z0 != 0) + (z1 != 0) + (z2 != 0);
+  return test > ncmp;
+}
+
+__device__ __forceinline__ int winIndex(int z0, int z1, int z2, int W) {
+  const int n = 2 * W + 1;
+  return ((z0 + W) * n + (z1 + W)) * n + (z2 + W);
+}
+
+template <bool WRITE>
+__global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int t = blockIdx.x * kWaves + (threadIdx.x >> 6);
+  if (t >= a.nt) return;  // uniform per wavefront
+  const int i = a.tcell[t], p = a.tpar[t];
+  const int eb = a.eBeg[p], ee = a.eBeg[p + 1];
+  const int ti0 = a.T.idx[3 * i], ti1 = a.T.idx[3 * i + 1], ti2 = a.T.idx[3 * i + 2];
+  const double tc0 = a.T.ctr[3 * i], tc1 = a.T.ctr[3 * i + 1], tc2 = a.T.ctr[3 * i + 2];
+  const bool tleaf = a.T.nson[i] == 0;
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  int nM = 0, nP = 0, nE = 0;
+  int bM = 0, bP = 0, bE = 0;
+  if (WRITE) {
+    bM = a.oM[t];
+    bP = a.oP[t];
+    bE = a.oE[t];
+  }
+  for (int e0 = eb; e0 < ee; e0 += 8) {
+    const int e = e0 + (lane >> 3), b = lane & 7;
+    int j = -1;
+    if (e < ee) {
+      const int q = a.eSrc[e];
+      if (b < a.S.nson[q]) j = a.S.sons[a.S.son0[q] + b];
+    }
+    int cls = 0, z0 = 0, z1 = 0, z2 = 0;
+    if (j >= 0) {
+      z0 = ti0 - a.S.idx[3 * j];
+      z1 = ti1 - a.S.idx[3 * j + 1];
+      z2 = ti2 - a.S.idx[3 * j + 2];
+      bool far = false;
+      if (a.kind == 1) far = macHF(tc0, tc1, tc2, a.S.ctr + 3 * j, a.S.rad[j], a.kappa);
+      else if (a.kind == 2) far = macLF(z0, z1, z2);
+      cls = far ? 1 : (tleaf || a.S.nson[j] == 0) ? 2 : 3;
+    }
+    const unsigned long long mM = __ballot(cls == 1), mP = __ballot(cls == 2),
+                             mE = __ballot(cls == 3);
+    if (WRITE) {
+      if (cls == 1) {
+        a.mSrc[bM + nM + __popcll(mM & lt)] = j;
+        const int W = a.W;
+        if (abs(z0) <= W && abs(z1) <= W && abs(z2) <= W) {
+          unsigned long long* r = a.rep + winIndex(z0, z1, z2, W);
+          const unsigned long long v = ((unsigned long long)(unsigned)i << 32) | (unsigned)j;
+          if (v < __atomic_load_n(r, __ATOMIC_RELAXED)) atomicMin(r, v);
+        } else {
+          atomicOr(a.overflow, 1);
+        }
+      } else if (cls == 2) {
+        a.pSrc[bP + nP + __popcll(mP & lt)] = j;
+      } else if (cls == 3) {
+        a.eOut[bE + nE + __popcll(mE & lt)] = j;
+      }
+    }
+    nM += __popcll(mM);
+    nP += __popcll(mP);
+    nE += __popcll(mE);
+  }
+  if (!WRITE && lane == 0) {
+    a.cM[t] = nM;
+    a.cP[t] = nP;
+    a.cE[t] = nE;
+  }
+}
+
+// The M2L pairs of one level, CSR by target cell (the level's target list).
+struct PairArgs {
+  int nt;
+  const int* tcell;
+  const int* mBeg;
+  const int* mSrc;
+  const int* Tidx;
+  const int* Sidx;
+  const unsigned long long* code;  // per window offset: id | hash << 32 | (cone + 1) << 40
+  int W;
+};
+
+// Effective directions (M2L.hpp:189-197): both cells of an HF pair need the
+// pair's cone.
+__global__ __launch_bounds__(256) void k_marks(PairArgs a, const int64_t* Tmark0,
+                                               const int64_t* Smark0, uint8_t* Tm, uint8_t* Sm) {
+  const int lane = threadIdx.x & 63;
+  const int t = blockIdx.x * kWaves + (threadIdx.x >> 6);
+  if (t >= a.nt) return;
+  const int i = a.tcell[t];
+  const int ti0 = a.Tidx[3 * i], ti1 = a.Tidx[3 * i + 1], ti2 = a.Tidx[3 * i + 2];
+  const int64_t tm = Tmark0[i];
+  for (int e = a.mBeg[t] + lane; e < a.mBeg[t + 1]; e += 64) {
+    const int j = a.mSrc[e];
+    const unsigned long long c =
+        a.code[winIndex(ti0 - a.Sidx[3 * j], ti1 - a.Sidx[3 * j + 1], ti2 - a.Sidx[3 * j + 2], a.W)];
+    const int cone = (int)(c >> 40) - 1;
+    Tm[tm + cone] = 1;
+    Sm[Smark0[j] + cone] = 1;
+  }
+}
+
+struct DSlots {
+  const int* slot0;
+  const int* nslot;
+  const uint8_t* lf;     // the cell's single slot is an LF slot
+  const int64_t* mark0;
+  const int* slotIdx;    // per (cell, direction) of the marks
+};
+
+// Tree::slotOf
+__device__ __forceinline__ int slotOf(const DSlots& s, int c, int d) {
+  if (s.nslot[c] == 0) return -1;
+  if (s.lf[c]) return s.slot0[c];
+  const int64_t m = s.mark0[c];
+  return (d >= 0 && m >= 0) ? s.slotIdx[m + d] : -1;
+}
+
+struct ListArgs {
+  PairArgs pa;
+  int hf;
+  DSlots TS, SS;
+  int* cnt;          // per target slot
+  uint8_t* used;     // per source slot
+  int* missing;
+  int* cur;          // per target slot: next free entry
+  const int* m2fIndex;
+  int* outSrcF;
+  int* outTab;
+  int* outHash;
+};
+
+__device__ __forceinline__ void pairSlots(const ListArgs& a, int i, int ti0, int ti1, int ti2,
+                                          int j, int& ts, int& ss, int& id, int& h) {
+  const PairArgs& p = a.pa;
+  const unsigned long long c =
+      p.code[winIndex(ti0 - p.Sidx[3 * j], ti1 - p.Sidx[3 * j + 1], ti2 - p.Sidx[3 * j + 2], p.W)];
+  const int cone = a.hf ? (int)(c >> 40) - 1 : -1;
+  id = (int)(c & 0xFFFFFFFFu);
+  h = (int)((c >> 32) & 63);
+  ts = slotOf(a.TS, i, cone);
+  ss = slotOf(a.SS, j, cone);
+}
+
+__global__ __launch_bounds__(256) void k_list_count(ListArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int t = blockIdx.x * kWaves + (threadIdx.x >> 6);
+  if (t >= a.pa.nt) return;
+  const int i = a.pa.tcell[t];
+  const int ti0 = a.pa.Tidx[3 * i], ti1 = a.pa.Tidx[3 * i + 1], ti2 = a.pa.Tidx[3 * i + 2];
+  for (int e = a.pa.mBeg[t] + lane; e < a.pa.mBeg[t + 1]; e += 64) {
+    int ts, ss, id, h;
+    pairSlots(a, i, ti0, ti1, ti2, a.pa.mSrc[e], ts, ss, id, h);
+    if (ts < 0 || ss < 0) {
+      atomicAdd(a.missing, 1);
+    } else {
+      atomicAdd(a.cnt + ts, 1);
+      a.used[ss] = 1;
+    }
+  }
+}
+
+// Stable placement: the pairs of a target cell in segment order, each at its
+// target slot's cursor.  Per 64-pair chunk, the lanes that share the first
+// unplaced lane's slot take consecutive entries (one atomic on the cursor,
+// whose returned value orders successive chunks), until every lane is placed.
+__global__ __launch_bounds__(256) void k_list_place(ListArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int t = blockIdx.x * kWaves + (threadIdx.x >> 6);
+  if (t >= a.pa.nt) return;
+  const int i = a.pa.tcell[t];
+  const int ti0 = a.pa.Tidx[3 * i], ti1 = a.pa.Tidx[3 * i + 1], ti2 = a.pa.Tidx[3 * i + 2];
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  const int mb = a.pa.mBeg[t], me = a.pa.mBeg[t + 1];
+  for (int e0 = mb; e0 < me; e0 += 64) {
+    const int e = e0 + lane;
+    const bool v = e < me;
+    int ts = -1, ss = -1, id = 0, h = 0;
+    if (v) pairSlots(a, i, ti0, ti1, ti2, a.pa.mSrc[e], ts, ss, id, h);
+    unsigned long long todo = __ballot(v);
+    int pos = 0;
+    while (todo) {
+      const int lead = __ffsll((long long)todo) - 1;
+      const int s0 = __shfl(ts, lead);
+      const unsigned long long m = __ballot(v && ts == s0) & todo;
+      int base = 0;
+      if (lane == lead) base = atomicAdd(a.cur + s0, __popcll(m));
+      base = __shfl(base, lead);
+      if ((m >> lane) & 1ull) pos = base + __popcll(m & lt);
+      todo &= ~m;
+    }
+    if (v) {
+      a.outSrcF[pos] = a.m2fIndex[ss];
+      a.outTab[pos] = id;
+      a.outHash[pos] = h;
+    }
+  }
+}
+
+class GpuSetup final : public SetupDevice {
+ public:
+  ~GpuSetup() override {
+    for (void* p : owned_) (void)hipFree(p);
+  }
+  int walk(Plan& pl, std::vector<std::array<int, 4>>& tabKeys, std::vector<int>& pt,
+           std::vector<int>& ps, std::string& msg) override;
+  int m2lLists(Plan& pl, std::string& msg) override;
+
+ private:
+  struct Level {
+    int l = 0, nt = 0, kind = 0, W = -1;
+    int* tcell = nullptr;
+    int* mBeg = nullptr;   // nt + 1
+    int* mSrc = nullptr;
+    unsigned long long* code = nullptr;
+    int64_t nm = 0;
+  };
+  std::vector<Level> lv_;
+  std::vector<void*> owned_;
+  int* Tidx_ = nullptr;
+  int* Sidx_ = nullptr;
+
+  template <class T>
+  hipError_t alloc(T*& p, size_t n) {
+    p = nullptr;
+    if (n == 0) return hipSuccess;
+    hipError_t e = hipMalloc((void**)&p, n * sizeof(T));
+    if (e == hipSuccess) owned_.push_back((void*)p);
+    else p = nullptr;
+    return e;
+  }
+  template <class T>
+  hipError_t put(T*& d, const T* h, size_t n) {
+    hipError_t e = alloc(d, n);
+    if (e != hipSuccess || n == 0) return e;
+    return hipMemcpy(d, h, n * sizeof(T), hipMemcpyHostToDevice);
+  }
+  template <class T>
+  hipError_t put(T*& d, const std::vector<T>& h) {
+    return put(d, h.data(), h.size());
+  }
+  void release(void* p) {
+    if (!p) return;
+    for (size_t k = 0; k < owned_.size(); k++)
+      if (owned_[k] == p) {
+        (void)hipFree(p);
+        owned_[k] = owned_.back();
+        owned_.pop_back();
+        return;
+      }
+  }
+};
+
+// A failed HIP call hands the plan back to the host builder (the error is
+// consumed, so that later calls do not report it).
+#define SCK(call)                                                                   \
+  do {                                                                              \
+    hipError_t e_ = (call);                                                         \
+    if (e_ != hipSuccess) {                                                         \
+      (void)hipGetLastError();                                                      \
+      msg = std::string("device set-up: ") + hipGetErrorString(e_) + " at " #call; \
+      return kSetupFallback;                                                        \
+    }                                                                               \
+  } while (0)
+
+inline unsigned blocksFor(int nt) { return (unsigned)((nt + kWaves - 1) / kWaves); }
+
+int GpuSetup::walk(Plan& pl, std::vector<std::array<int, 4>>& tabKeys, std::vector<int>& pt,
+                   std::vector<int>& ps, std::string& msg) {
+  const Tree &T = pl.T, &S = pl.S;
+  std::vector<int> W(pl.nlvl, -1);
+  for (int l = 0; l < pl.nlvl; l++) {
+    W[l] = m2lWindow(pl, l);
+    if (pl.precomp[l] && W[l] < 0) {
+      msg = "device set-up: offset window too large";
+      return kSetupFallback;
+    }
+  }
+  pt.clear();
+  ps.clear();
+  int64_t nHF = 0, nLF = 0, nP2P = 0;
+  // the root pair (level 0 has no M2L)
+  if (T.leaf(0) || S.leaf(0)) {
+    pt.push_back(0);
+    ps.push_back(0);
+    pl.st.hf_m2l = pl.st.lf_m2l = 0;
+    pl.st.p2p_calls = 1;
+    pl.st.p2p_pairs = (int64_t)T.nprt[0] * S.nprt[0];
+    tabKeys.clear();
+    return OK;
+  }
+  // target cells per level (cell order) and their parents' positions
+  const int nT = T.ncells();
+  std::vector<int> pos(nT, 0), parent(nT, -1);
+  std::vector<std::vector<int>> cells(pl.nlvl);
+  for (int c = 0; c < nT; c++) {
+    pos[c] = (int)cells[T.lvl[c]].size();
+    cells[T.lvl[c]].push_back(c);
+    for (int a = 0; a < T.nson[c]; a++) parent[T.sons[T.son0[c] + a]] = c;
+  }
+  double *Tctr = nullptr, *Sctr = nullptr, *Srad = nullptr;
+  int *Tnson = nullptr, *Snson = nullptr, *Sson0 = nullptr, *Ssons = nullptr;
+  SCK(put(Tidx_, T.idx));
+  SCK(put(Tctr, T.ctr));
+  SCK(put(Tnson, T.nson));
+  SCK(put(Sidx_, S.idx));
+  SCK(put(Sctr, S.ctr));
+  SCK(put(Srad, S.rad));
+  SCK(put(Snson, S.nson));
+  SCK(put(Sson0, S.son0));
+  SCK(put(Ssons, S.sons));
+  int* overflow = nullptr;
+  SCK(alloc(overflow, 1));
+  SCK(hipMemset(overflow, 0, sizeof(int)));
+  std::vector<unsigned long long*> rep(pl.nlvl, nullptr);
+  for (int l = 0; l < pl.nlvl; l++) {
+    if (W[l] < 0) continue;
+    const size_t n = (size_t)(2 * W[l] + 1) * (2 * W[l] + 1) * (2 * W[l] + 1);
+    SCK(alloc(rep[l], n));
+    SCK(hipMemset(rep[l], 0xFF, n * sizeof(unsigned long long)));
+  }
+  // level 0: the root target expands the root source
+  int *eBeg = nullptr, *eSrc = nullptr;
+  {
+    const int b[2] = {0, 1}, s0 = 0;
+    SCK(put(eBeg, b, 2));
+    SCK(put(eSrc, &s0, 1));
+  }
+  int64_t nE = 1, nmTotal = 0;
+  const DTree dT = {Tidx_, Tctr, nullptr, nullptr, Tnson, nullptr};
+  const DTree dS = {Sidx_, Sctr, Srad, Sson0, Snson, Ssons};
+  for (int l = 1; l < pl.nlvl && nE > 0; l++) {
+    const int nt = (int)cells[l].size();
+    if (nt == 0) break;
+    std::vector<int> tpar(nt);
+    for (int t = 0; t < nt; t++) tpar[t] = pos[parent[cells[l][t]]];
+    Level L;
+    L.l = l;
+    L.nt = nt;
+    L.kind = pl.precomp[l] ? ((l && pl.isHF[l]) ? 1 : 2) : 0;
+    L.W = W[l];
+    int *tparD = nullptr, *cnt = nullptr;
+    SCK(put(L.tcell, cells[l]));
+    SCK(put(tparD, tpar));
+    SCK(alloc(cnt, (size_t)3 * nt));
+    WalkArgs a = {};
+    a.T = dT;
+    a.S = dS;
+    a.nt = nt;
+    a.tcell = L.tcell;
+    a.tpar = tparD;
+    a.eBeg = eBeg;
+    a.eSrc = eSrc;
+    a.kind = L.kind;
+    a.kappa = pl.kappa;
+    a.cM = cnt;
+    a.cP = cnt + nt;
+    a.cE = cnt + 2 * (size_t)nt;
+    a.rep = rep[l];
+    a.W = L.W;
+    a.overflow = overflow;
+    k_walk<false><<<blocksFor(nt), 256, 0, 0>>>(a);
+    SCK(hipGetLastError());
+    std::vector<int> hc((size_t)3 * nt);
+    SCK(hipMemcpy(hc.data(), cnt, hc.size() * sizeof(int), hipMemcpyDeviceToHost));
+    release(cnt);
+    // exclusive sums: oM | oP | oE, nt + 1 each
+    std::vector<int> off((size_t)3 * (nt + 1));
+    int64_t tot[3] = {0, 0, 0};
+    for (int k = 0; k < 3; k++) {
+      for (int t = 0; t < nt; t++) {
+        off[(size_t)k * (nt + 1) + t] = (int)tot[k];
+        tot[k] += hc[(size_t)k * nt + t];
+        if (tot[k] >= ((int64_t)1 << 31)) {
+          msg = "device set-up: more than 2^31 pairs at one level";
+          return kSetupFallback;
+        }
+      }
+      off[(size_t)k * (nt + 1) + nt] = (int)tot[k];
+    }
+    int* offD = nullptr;
+    SCK(put(offD, off));
+    SCK(alloc(L.mSrc, (size_t)tot[0]));
+    int *pSrc = nullptr, *eOut = nullptr;
+    SCK(alloc(pSrc, (size_t)tot[1]));
+    SCK(alloc(eOut, (size_t)tot[2]));
+    a.oM = offD;
+    a.oP = offD + (nt + 1);
+    a.oE = offD + 2 * (size_t)(nt + 1);
+    a.mSrc = L.mSrc;
+    a.pSrc = pSrc;
+    a.eOut = eOut;
+    k_walk<true><<<blocksFor(nt), 256, 0, 0>>>(a);
+    SCK(hipGetLastError());
+    // P2P pairs to the host, target cells from the segments
+    if (tot[1]) {
+      const size_t p0 = ps.size();
+      ps.resize(p0 + (size_t)tot[1]);
+      SCK(copyLarge(ps.data() + p0, pSrc, (size_t)tot[1] * sizeof(int), false));
+      pt.resize(ps.size());
+      for (int t = 0; t < nt; t++)
+        for (int q = off[(size_t)(nt + 1) + t]; q < off[(size_t)(nt + 1) + t + 1]; q++)
+          pt[p0 + q] = cells[l][t];
+    }
+    release(pSrc);
+    release(tparD);
+    // M2L segments: keep the offsets on both sides
+    SCK(alloc(L.mBeg, (size_t)nt + 1));
+    SCK(hipMemcpy(L.mBeg, offD, ((size_t)nt + 1) * sizeof(int), hipMemcpyDeviceToDevice));
+    release(offD);
+    L.nm = tot[0];
+    if (L.kind == 1) nHF += tot[0];
+    if (L.kind == 2) nLF += tot[0];
+    nP2P += tot[1];
+    nmTotal += tot[0];
+    // the next level expands this level's expansions
+    release(eBeg);
+    release(eSrc);
+    std::vector<int> eb(off.begin() + 2 * (size_t)(nt + 1), off.end());
+    SCK(put(eBeg, eb));
+    eSrc = eOut;
+    nE = tot[2];
+    if (L.nm) lv_.push_back(L);
+    else {
+      release(L.tcell);
+      release(L.mSrc);
+      release(L.mBeg);
+    }
+  }
+  release(eBeg);
+  release(eSrc);
+  int ovf = 0;
+  SCK(hipMemcpy(&ovf, overflow, sizeof(int), hipMemcpyDeviceToHost));
+  if (ovf) {
+    msg = "device set-up: an M2L offset outside the level's window";
+    return kSetupFallback;
+  }
+  if (nmTotal >= ((int64_t)1 << 31)) {
+    msg = "M2L lists exceed 2^31 entries";
+    return ERR_MEMORY;
+  }
+  // table ids, hashes and cones of the offsets met
+  std::vector<std::vector<uint64_t>> hrep(pl.nlvl), code;
+  for (int l = 0; l < pl.nlvl; l++) {
+    if (!rep[l]) continue;
+    const size_t n = (size_t)(2 * W[l] + 1) * (2 * W[l] + 1) * (2 * W[l] + 1);
+    hrep[l].resize(n);
+    SCK(hipMemcpy(hrep[l].data(), rep[l], n * sizeof(uint64_t), hipMemcpyDeviceToHost));
+  }
+  int r = offsetCodes(pl, W, hrep, code, tabKeys, msg);
+  if (r != OK) return r;
+  for (int l = 0; l < pl.nlvl; l++) {
+    if (!rep[l]) continue;
+    SCK(hipMemcpy(rep[l], code[l].data(), code[l].size() * sizeof(uint64_t),
+                  hipMemcpyHostToDevice));
+  }
+  for (Level& L : lv_) L.code = rep[L.l];
+  // effective-direction marks of both trees
+  {
+    uint8_t *Tm = nullptr, *Sm = nullptr;
+    int64_t *Tmark0 = nullptr, *Smark0 = nullptr;
+    SCK(alloc(Tm, T.marks.size()));
+    SCK(alloc(Sm, S.marks.size()));
+    if (Tm) SCK(hipMemset(Tm, 0, T.marks.size()));
+    if (Sm) SCK(hipMemset(Sm, 0, S.marks.size()));
+    SCK(put(Tmark0, T.mark0));
+    SCK(put(Smark0, S.mark0));
+    for (const Level& L : lv_) {
+      if (L.kind != 1) continue;
+      PairArgs pa = {L.nt, L.tcell, L.mBeg, L.mSrc, Tidx_, Sidx_, L.code, L.W};
+      k_marks<<<blocksFor(L.nt), 256, 0, 0>>>(pa, Tmark0, Smark0, Tm, Sm);
+      SCK(hipGetLastError());
+    }
+    if (Tm) SCK(copyLarge(pl.T.marks.data(), Tm, T.marks.size(), false));
+    if (Sm) SCK(copyLarge(pl.S.marks.data(), Sm, S.marks.size(), false));
+    release(Tm);
+    release(Sm);
+    release(Tmark0);
+    release(Smark0);
+  }
+  release(Tctr);
+  release(Sctr);
+  release(Srad);
+  release(Tnson);
+  release(Snson);
+  release(Sson0);
+  release(Ssons);
+  int64_t pairs = 0;
+  const int64_t np = (int64_t)pt.size();
+  for (int64_t k = 0; k < np; k++) pairs += (int64_t)T.nprt[pt[k]] * S.nprt[ps[k]];
+  pl.st.hf_m2l = nHF;
+  pl.st.lf_m2l = nLF;
+  pl.st.p2p_calls = nP2P;
+  pl.st.p2p_pairs = pairs;
+  return OK;
+}
+
+int GpuSetup::m2lLists(Plan& pl, std::string& msg) {
+  const Tree &T = pl.T, &S = pl.S;
+  const int nts = T.nslots, nss = S.nslots;
+  int64_t nm = 0;
+  for (const Level& L : lv_) nm += L.nm;
+  pl.m2lBeg.assign((size_t)nts + 1, 0);
+  pl.m2fIndex.assign(nss, -1);
+  pl.m2fSlot.clear();
+  pl.m2lSrcF.clear();
+  pl.m2lTabId.clear();
+  pl.m2lHash.clear();
+  if (nm == 0) return OK;
+  auto upSlots = [&](const Tree& t, DSlots& d) -> hipError_t {
+    std::vector<uint8_t> lf(t.ncells(), 0);
+    for (int c = 0; c < t.ncells(); c++) lf[c] = t.nslot[c] > 0 && t.slotDir[t.slot0[c]] < 0;
+    int *s0 = nullptr, *ns = nullptr, *si = nullptr;
+    uint8_t* lfD = nullptr;
+    int64_t* m0 = nullptr;
+    hipError_t e;
+    if ((e = put(s0, t.slot0)) != hipSuccess) return e;
+    if ((e = put(ns, t.nslot)) != hipSuccess) return e;
+    if ((e = put(lfD, lf)) != hipSuccess) return e;
+    if ((e = put(m0, t.mark0)) != hipSuccess) return e;
+    if ((e = put(si, t.slotIdx)) != hipSuccess) return e;
+    d = {s0, ns, lfD, m0, si};
+    return hipSuccess;
+  };
+  ListArgs a = {};
+  SCK(upSlots(T, a.TS));
+  SCK(upSlots(S, a.SS));
+  int* missing = nullptr;
+  SCK(alloc(a.cnt, (size_t)nts));
+  SCK(alloc(a.used, (size_t)nss));
+  SCK(alloc(missing, 1));
+  SCK(hipMemset(a.cnt, 0, (size_t)nts * sizeof(int)));
+  SCK(hipMemset(a.used, 0, (size_t)nss));
+  SCK(hipMemset(missing, 0, sizeof(int)));
+  a.missing = missing;
+  for (const Level& L : lv_) {
+    a.pa = {L.nt, L.tcell, L.mBeg, L.mSrc, Tidx_, Sidx_, L.code, L.W};
+    a.hf = L.kind == 1;
+    k_list_count<<<blocksFor(L.nt), 256, 0, 0>>>(a);
+    SCK(hipGetLastError());
+  }
+  int nmiss = 0;
+  SCK(hipMemcpy(&nmiss, missing, sizeof(int), hipMemcpyDeviceToHost));
+  if (nmiss) {
+    msg = "internal: missing M2L slot";
+    return ERR_STATE;
+  }
+  {
+    std::vector<int> c(nts);
+    SCK(hipMemcpy(c.data(), a.cnt, (size_t)nts * sizeof(int), hipMemcpyDeviceToHost));
+    int64_t run = 0;
+    for (int s = 0; s < nts; s++) {
+      run += c[s];
+      pl.m2lBeg[s + 1] = (int)run;
+    }
+    if (run != nm) {
+      msg = "internal: M2L entry count mismatch";
+      return ERR_STATE;
+    }
+    std::vector<uint8_t> used(nss);
+    SCK(hipMemcpy(used.data(), a.used, (size_t)nss, hipMemcpyDeviceToHost));
+    for (int s = 0; s < nss; s++)
+      if (used[s]) {
+        pl.m2fIndex[s] = (int)pl.m2fSlot.size();
+        pl.m2fSlot.push_back(s);
+      }
+  }
+  release(a.used);
+  a.used = nullptr;
+  // cursors start at the slots' first entries (the counts are no longer needed)
+  SCK(hipMemcpy(a.cnt, pl.m2lBeg.data(), (size_t)nts * sizeof(int), hipMemcpyHostToDevice));
+  a.cur = a.cnt;
+  int* f = nullptr;
+  SCK(put(f, pl.m2fIndex));
+  a.m2fIndex = f;
+  SCK(alloc(a.outSrcF, (size_t)nm));
+  SCK(alloc(a.outTab, (size_t)nm));
+  SCK(alloc(a.outHash, (size_t)nm));
+  for (const Level& L : lv_) {
+    a.pa = {L.nt, L.tcell, L.mBeg, L.mSrc, Tidx_, Sidx_, L.code, L.W};
+    a.hf = L.kind == 1;
+    k_list_place<<<blocksFor(L.nt), 256, 0, 0>>>(a);
+    SCK(hipGetLastError());
+  }
+  pl.m2lSrcF.resize(nm);
+  pl.m2lTabId.resize(nm);
+  pl.m2lHash.resize(nm);
+  SCK(copyLarge(pl.m2lSrcF.data(), a.outSrcF, (size_t)nm * sizeof(int), false));
+  SCK(copyLarge(pl.m2lTabId.data(), a.outTab, (size_t)nm * sizeof(int), false));
+  SCK(copyLarge(pl.m2lHash.data(), a.outHash, (size_t)nm * sizeof(int), false));
+  return OK;
+}
+
+#undef SCK
+
+}  // namespace
+
+SetupDevice* newSetupDevice() { return new GpuSetup(); }
+
+namespace {
+// memcpy split over host threads
+void parallelCopy(char* dst, const char* src, size_t n) {
+  const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  const size_t per = std::max<size_t>((size_t)4 << 20, (n + hw - 1) / hw);
+  std::vector<std::thread> th;
+  for (size_t o = per; o < n; o += per)
+    th.emplace_back([=] { std::memcpy(dst + o, src + o, std::min(per, n - o)); });
+  std::memcpy(dst, src, std::min(per, n));
+  for (auto& t : th) t.join();
+}
+}  // namespace
+
+hipError_t copyLarge(void* dst, const void* src, size_t bytes, bool toDevice) {
+  const hipMemcpyKind kind = toDevice ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost;
+  const size_t chunk = (size_t)64 << 20;
+  if (bytes < chunk) return hipMemcpy(dst, src, bytes, kind);
+  hipError_t e = hipDeviceSynchronize();
+  if (e != hipSuccess) return e;
+  void* pin[2] = {nullptr, nullptr};
+  hipStream_t s = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  bool staged = hipHostMalloc(&pin[0], chunk, hipHostMallocDefault) == hipSuccess &&
+                hipHostMalloc(&pin[1], chunk, hipHostMallocDefault) == hipSuccess &&
+                hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess &&
+                hipEventCreateWithFlags(&ev[0], hipEventDisableTiming) == hipSuccess &&
+                hipEventCreateWithFlags(&ev[1], hipEventDisableTiming) == hipSuccess;
+  if (!staged) {
+    (void)hipGetLastError();
+    e = hipMemcpy(dst, src, bytes, kind);
+  } else {
+    const size_t nch = (bytes + chunk - 1) / chunk;
+    auto len = [&](size_t k) { return std::min(chunk, bytes - k * chunk); };
+    char* d = (char*)dst;
+    const char* h = (const char*)src;
+    if (toDevice) {
+      // chunk k: host -> pin[k % 2] (after chunk k - 2's device copy), pin -> device
+      for (size_t k = 0; k < nch && e == hipSuccess; k++) {
+        const int b = (int)(k & 1);
+        if (k >= 2) e = hipEventSynchronize(ev[b]);
+        if (e != hipSuccess) break;
+        parallelCopy((char*)pin[b], h + k * chunk, len(k));
+        e = hipMemcpyAsync(d + k * chunk, pin[b], len(k), kind, s);
+        if (e == hipSuccess) e = hipEventRecord(ev[b], s);
+      }
+    } else {
+      // chunk k: device -> pin[k % 2]; chunk k + 1 is in flight while chunk k
+      // moves from pin to host
+      e = hipMemcpyAsync(pin[0], h, len(0), kind, s);
+      if (e == hipSuccess) e = hipEventRecord(ev[0], s);
+      for (size_t k = 0; k < nch && e == hipSuccess; k++) {
+        const int b = (int)(k & 1);
+        e = hipEventSynchronize(ev[b]);
+        if (e == hipSuccess && k + 1 < nch) {
+          e = hipMemcpyAsync(pin[b ^ 1], h + (k + 1) * chunk, len(k + 1), kind, s);
+          if (e == hipSuccess) e = hipEventRecord(ev[b ^ 1], s);
+        }
+        if (e == hipSuccess) parallelCopy(d + k * chunk, (const char*)pin[b], len(k));
+      }
+    }
+    hipError_t e2 = hipStreamSynchronize(s);
+    if (e == hipSuccess) e = e2;
+  }
+  for (hipEvent_t x : ev)
+    if (x) (void)hipEventDestroy(x);
+  if (s) (void)hipStreamDestroy(s);
+  for (void* p : pin)
+    if (p) (void)hipHostFree(p);
+  return e;
+}
+
+}  // namespace dfm

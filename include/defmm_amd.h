@@ -63,6 +63,22 @@ int defmm_plan_create(const double* src_xyz, int64_t n_src, const double* trg_xy
                       int precision, defmm_plan* out);
 
 /*
+ * Where defmm_plan_create runs the dual walk (the reference's set-up
+ * traversal, include/traversals/hrzpass.hpp:111-138, and its M2L list
+ * building, include/interface.hpp:100-151) and the M2L lists:
+ * DEFMM_SETUP_DEVICE (the default) on the current GPU, DEFMM_SETUP_HOST in
+ * host C++ (OpenMP).  Both build the same plan, entry for entry.  The device
+ * path hands a plan it cannot take (an M2L offset window above 2^26
+ * offsets, device memory exhausted) to the host path.  No counterpart in the
+ * reference (its set-up is host code); for comparing the two builders.
+ * Process-wide; returns the previous mode, or DEFMM_EARG negated for an
+ * unknown mode.
+ */
+#define DEFMM_SETUP_DEVICE 0
+#define DEFMM_SETUP_HOST 1
+int defmm_set_setup_mode(int mode);
+
+/*
  * Evaluation: replaces gemv(A, In, Out) (reference include/interface.hpp:
  * 153-170).  q: n_src complex charges (interleaved re, im doubles, input
  * order, host memory); p: n_trg complex potentials, overwritten (host).
@@ -295,6 +311,8 @@ enum {
   /* rows of the M2L kernel-table store on the device (one per table, axis
      order and sign flips in use) */
   DEFMM_STAT_M2L_TABLE_ROWS,
+  /* 1 when the dual walk and the M2L lists were built on the GPU */
+  DEFMM_STAT_SETUP_DEVICE,
   DEFMM_NSTATS
 };
 
