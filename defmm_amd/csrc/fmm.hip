@@ -2645,7 +2645,15 @@ static int upload(Exec& X, Plan& pl) {
   X.hGroupBeg = pl.m2lGroupBeg;
   {
     std::vector<int> th(pl.m2lTabId.size());
-    for (size_t k = 0; k < th.size(); k++) th[k] = (pl.m2lTabId[k] << 6) | pl.m2lHash[k];
+    const size_t n = th.size();
+    const unsigned nth = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> ts;
+    for (unsigned t = 0; t < nth; t++)
+      ts.emplace_back([&, t] {
+        for (size_t k = n * t / nth; k < n * (t + 1) / nth; k++)
+          th[k] = (pl.m2lTabId[k] << 6) | pl.m2lHash[k];
+      });
+    for (auto& x : ts) x.join();
     CKR(X.upv(X.m2lTabHash, th));
   }
   // P2P work items: 64-target chunks x runs of <= budget sources.  The

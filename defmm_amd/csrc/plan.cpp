@@ -1346,10 +1346,20 @@ int validatePlan(const Plan& pl, std::string& msg) {
   for (int s : pl.m2fSlot)
     if (s < 0 || s >= pl.S.nslots) return bad("m2f slot");
   if ((int)pl.m2lBeg.size() != pl.T.nslots + 1 || pl.m2lBeg.back() != (int)pl.m2lSrcF.size()) return bad("m2l csr");
-  for (size_t e = 0; e < pl.m2lSrcF.size(); e++) {
-    if (pl.m2lSrcF[e] < 0 || pl.m2lSrcF[e] >= (int)pl.m2fSlot.size()) return bad("m2l source");
-    if (pl.m2lTabId[e] < 0 || pl.m2lTabId[e] >= pl.ntab || pl.m2lTabId[e] >= (1 << 25)) return bad("m2l table");
-    if (pl.m2lHash[e] < 0 || pl.m2lHash[e] > 63) return bad("m2l hash");
+  {
+    // the entry checks in parallel, failures as bits
+    const int64_t nm = (int64_t)pl.m2lSrcF.size();
+    const int nf = (int)pl.m2fSlot.size();
+    int fail = 0;
+#pragma omp parallel for schedule(static) reduction(| : fail)
+    for (int64_t e = 0; e < nm; e++) {
+      if (pl.m2lSrcF[e] < 0 || pl.m2lSrcF[e] >= nf) fail |= 1;
+      if (pl.m2lTabId[e] < 0 || pl.m2lTabId[e] >= pl.ntab || pl.m2lTabId[e] >= (1 << 25)) fail |= 2;
+      if (pl.m2lHash[e] < 0 || pl.m2lHash[e] > 63) fail |= 4;
+    }
+    if (fail & 1) return bad("m2l source");
+    if (fail & 2) return bad("m2l table");
+    if (fail & 4) return bad("m2l hash");
   }
   for (int v : pl.perm)
     if (v < 0 || v >= pl.PP) return bad("permutation");
@@ -1368,6 +1378,7 @@ int validatePlan(const Plan& pl, std::string& msg) {
         pl.m2lGCode.size() != pl.m2lGSrcF.size() * GG)
       return bad("m2l group csr");
     std::vector<char> seen(pl.T.nslots, 0);
+    std::vector<int> haveOf(ng, 0);
     for (size_t g = 0; g < ng; g++) {
       if (pl.m2lGroupBeg[g + 1] < pl.m2lGroupBeg[g]) return bad("m2l group order");
       int have = 0;
@@ -1380,17 +1391,26 @@ int validatePlan(const Plan& pl, std::string& msg) {
           have |= 1 << k;
         }
       }
+      haveOf[g] = have;
+    }
+    // the groups' entries in parallel, failures as bits
+    const int nf = (int)pl.m2fSlot.size();
+    int fail = 0;
+#pragma omp parallel for schedule(dynamic, 1024) reduction(| : fail)
+    for (int64_t g = 0; g < (int64_t)ng; g++)
       for (int e = pl.m2lGroupBeg[g]; e < pl.m2lGroupBeg[g + 1]; e++) {
-        if (pl.m2lGSrcF[e] < 0 || pl.m2lGSrcF[e] >= (int)pl.m2fSlot.size()) return bad("m2l group source");
+        if (pl.m2lGSrcF[e] < 0 || pl.m2lGSrcF[e] >= nf) fail |= 1;
         int m = pl.m2lGMask[e];
-        if (m <= 0 || (m & ~have)) return bad("m2l group mask");
+        if (m <= 0 || (m & ~haveOf[g])) fail |= 2;
         for (int k = 0; k < GG; k++) {
           if (!((m >> k) & 1)) continue;
           int c = pl.m2lGCode[(size_t)e * GG + k];
-          if ((c >> 6) < 0 || (c >> 6) >= pl.ntab) return bad("m2l group table");
+          if ((c >> 6) < 0 || (c >> 6) >= pl.ntab) fail |= 4;
         }
       }
-    }
+    if (fail & 1) return bad("m2l group source");
+    if (fail & 2) return bad("m2l group mask");
+    if (fail & 4) return bad("m2l group table");
     for (int v : pl.m2lOrder)
       if (!seen[v]) return bad("m2l target not in a group");
   }
