@@ -1136,56 +1136,6 @@ int buildPlan(Plan& pl, const double* src, int64_t ns, const double* trg, int64_
       gStart.push_back((int)pl.m2lOrder.size());
     }
     const int64_t ngroups = (int64_t)gStart.size() - 1;
-    // entries of contiguous ranges of groups built in parallel, then joined
-    // in range order (the result does not depend on the thread count)
-    const int64_t nchunk = std::max<int64_t>(1, std::min<int64_t>(ngroups, 1024));
-    struct Part {
-      std::vector<int> src, mask, code, nent;
-    };
-    std::vector<Part> parts(nchunk);
-#pragma omp parallel for schedule(dynamic, 1)
-    for (int64_t ch = 0; ch < nchunk; ch++) {
-      Part& pt = parts[ch];
-      std::vector<std::array<int, 3>> tmp;  // srcF, target in group, tabHash
-      const int64_t g0 = ngroups * ch / nchunk, g1 = ngroups * (ch + 1) / nchunk;
-      for (int64_t gi = g0; gi < g1; gi++) {
-        const int ng = gStart[gi + 1] - gStart[gi];
-        tmp.clear();
-        for (int g = 0; g < ng; g++) {
-          const int t = pl.m2lOrder[gStart[gi] + g];
-          for (int e = pl.m2lBeg[t]; e < pl.m2lBeg[t + 1]; e++)
-            tmp.push_back({pl.m2lSrcF[e], g, (pl.m2lTabId[e] << 6) | pl.m2lHash[e]});
-        }
-        std::stable_sort(tmp.begin(), tmp.end(),
-                         [](const std::array<int, 3>& a, const std::array<int, 3>& b) {
-                           return a[0] < b[0];
-                         });
-        int cnt = 0;
-        size_t k = 0;
-        while (k < tmp.size()) {
-          int src = tmp[k][0];
-          int mask = 0;
-          int codes[kM2LGroupMax] = {0};
-          size_t k2 = k;
-          // a target reads a source row at most once per group entry; a
-          // repeat (the same pair listed twice) starts a new entry
-          while (k2 < tmp.size() && tmp[k2][0] == src && !((mask >> tmp[k2][1]) & 1)) {
-            mask |= 1 << tmp[k2][1];
-            codes[tmp[k2][1]] = tmp[k2][2];
-            k2++;
-          }
-          pt.src.push_back(src);
-          pt.mask.push_back(mask);
-          for (int g = 0; g < G; g++) pt.code.push_back(codes[g]);
-          cnt++;
-          k = k2;
-        }
-        pt.nent.push_back(cnt);
-      }
-    }
-    int64_t total = 0;
-    for (const Part& pt : parts) total += (int64_t)pt.src.size();
-    if (total >= (int64_t)1 << 31) { msg = "M2L lists exceed 2^31 entries"; return ERR_MEMORY; }
     pl.m2lGroupTarget.resize(ngroups * G);
 #pragma omp parallel for schedule(static)
     for (int64_t gi = 0; gi < ngroups; gi++) {
@@ -1193,34 +1143,90 @@ int buildPlan(Plan& pl, const double* src, int64_t ns, const double* trg, int64_
       for (int g = 0; g < G; g++)
         pl.m2lGroupTarget[gi * G + g] = g < ng ? pl.m2lOrder[gStart[gi] + g] : -1;
     }
-    // the parts in range order: offsets first, then copied in parallel
-    std::vector<int64_t> eOff(nchunk + 1, 0), gOff(nchunk + 1, 0);
-    for (int64_t ch = 0; ch < nchunk; ch++) {
-      eOff[ch + 1] = eOff[ch] + (int64_t)parts[ch].src.size();
-      gOff[ch + 1] = gOff[ch] + (int64_t)parts[ch].nent.size();
-    }
-    pl.m2lGSrcF.resize(total);
-    pl.m2lGMask.resize(total);
-    pl.m2lGCode.resize(total * G);
-    pl.m2lGroupBeg.assign(ngroups + 1, 0);
-    {
-      // group starts: a running sum over the parts' entry counts
-      int64_t run = 0;
-      for (int64_t ch = 0; ch < nchunk; ch++)
-        for (size_t k = 0; k < parts[ch].nent.size(); k++) {
-          run += parts[ch].nent[k];
-          pl.m2lGroupBeg[gOff[ch] + (int64_t)k + 1] = (int)run;
-        }
-    }
+    if (dev) {
+      // a merge of the targets' (ascending) source rows per group, on the device
+      int r = dev->m2lGroups(pl, msg);
+      if (r != OK) return r;
+    } else {
+      // entries of contiguous ranges of groups built in parallel, then joined
+      // in range order (the result does not depend on the thread count)
+      const int64_t nchunk = std::max<int64_t>(1, std::min<int64_t>(ngroups, 1024));
+      struct Part {
+        std::vector<int> src, mask, code, nent;
+      };
+      std::vector<Part> parts(nchunk);
 #pragma omp parallel for schedule(dynamic, 1)
-    for (int64_t ch = 0; ch < nchunk; ch++) {
-      Part& pt = parts[ch];
-      std::copy(pt.src.begin(), pt.src.end(), pl.m2lGSrcF.begin() + eOff[ch]);
-      std::copy(pt.mask.begin(), pt.mask.end(), pl.m2lGMask.begin() + eOff[ch]);
-      std::copy(pt.code.begin(), pt.code.end(), pl.m2lGCode.begin() + eOff[ch] * G);
-      Part().src.swap(pt.src);
-      Part().mask.swap(pt.mask);
-      Part().code.swap(pt.code);
+      for (int64_t ch = 0; ch < nchunk; ch++) {
+        Part& pt = parts[ch];
+        std::vector<std::array<int, 3>> tmp;  // srcF, target in group, tabHash
+        const int64_t g0 = ngroups * ch / nchunk, g1 = ngroups * (ch + 1) / nchunk;
+        for (int64_t gi = g0; gi < g1; gi++) {
+          const int ng = gStart[gi + 1] - gStart[gi];
+          tmp.clear();
+          for (int g = 0; g < ng; g++) {
+            const int t = pl.m2lOrder[gStart[gi] + g];
+            for (int e = pl.m2lBeg[t]; e < pl.m2lBeg[t + 1]; e++)
+              tmp.push_back({pl.m2lSrcF[e], g, (pl.m2lTabId[e] << 6) | pl.m2lHash[e]});
+          }
+          std::stable_sort(tmp.begin(), tmp.end(),
+                           [](const std::array<int, 3>& a, const std::array<int, 3>& b) {
+                             return a[0] < b[0];
+                           });
+          int cnt = 0;
+          size_t k = 0;
+          while (k < tmp.size()) {
+            int src = tmp[k][0];
+            int mask = 0;
+            int codes[kM2LGroupMax] = {0};
+            size_t k2 = k;
+            // a target reads a source row at most once per group entry; a
+            // repeat (the same pair listed twice) starts a new entry
+            while (k2 < tmp.size() && tmp[k2][0] == src && !((mask >> tmp[k2][1]) & 1)) {
+              mask |= 1 << tmp[k2][1];
+              codes[tmp[k2][1]] = tmp[k2][2];
+              k2++;
+            }
+            pt.src.push_back(src);
+            pt.mask.push_back(mask);
+            for (int g = 0; g < G; g++) pt.code.push_back(codes[g]);
+            cnt++;
+            k = k2;
+          }
+          pt.nent.push_back(cnt);
+        }
+      }
+      int64_t total = 0;
+      for (const Part& pt : parts) total += (int64_t)pt.src.size();
+      if (total >= (int64_t)1 << 31) { msg = "M2L lists exceed 2^31 entries"; return ERR_MEMORY; }
+      // the parts in range order: offsets first, then copied in parallel
+      std::vector<int64_t> eOff(nchunk + 1, 0), gOff(nchunk + 1, 0);
+      for (int64_t ch = 0; ch < nchunk; ch++) {
+        eOff[ch + 1] = eOff[ch] + (int64_t)parts[ch].src.size();
+        gOff[ch + 1] = gOff[ch] + (int64_t)parts[ch].nent.size();
+      }
+      pl.m2lGSrcF.resize(total);
+      pl.m2lGMask.resize(total);
+      pl.m2lGCode.resize(total * G);
+      pl.m2lGroupBeg.assign(ngroups + 1, 0);
+      {
+        // group starts: a running sum over the parts' entry counts
+        int64_t run = 0;
+        for (int64_t ch = 0; ch < nchunk; ch++)
+          for (size_t k = 0; k < parts[ch].nent.size(); k++) {
+            run += parts[ch].nent[k];
+            pl.m2lGroupBeg[gOff[ch] + (int64_t)k + 1] = (int)run;
+          }
+      }
+#pragma omp parallel for schedule(dynamic, 1)
+      for (int64_t ch = 0; ch < nchunk; ch++) {
+        Part& pt = parts[ch];
+        std::copy(pt.src.begin(), pt.src.end(), pl.m2lGSrcF.begin() + eOff[ch]);
+        std::copy(pt.mask.begin(), pt.mask.end(), pl.m2lGMask.begin() + eOff[ch]);
+        std::copy(pt.code.begin(), pt.code.end(), pl.m2lGCode.begin() + eOff[ch] * G);
+        Part().src.swap(pt.src);
+        Part().mask.swap(pt.mask);
+        Part().code.swap(pt.code);
+      }
     }
     pl.st.m2l_groups = ngroups;
     pl.st.m2l_group_entries = (int64_t)pl.m2lGSrcF.size();

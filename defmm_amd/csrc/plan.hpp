@@ -156,14 +156,19 @@ struct Plan {
 // m2lLists() runs where the host builds the M2L lists: it fills m2fIndex,
 // m2fSlot and the CSR m2lBeg / m2lSrcF / m2lTabId / m2lHash, with the
 // entries of a target slot in ascending source cell order (the host walk's
-// order).  Either returns OK, an error code, or kSetupFallback when the
-// device cannot take this plan (the caller then builds it on the host).
+// order).  m2lGroups() builds the grouped M2L lists from them.  Each returns
+// OK, an error code, or kSetupFallback when the device cannot take this plan
+// (the caller then builds it on the host).
 constexpr int kSetupFallback = -1;
 struct SetupDevice {
   virtual ~SetupDevice() {}
   virtual int walk(Plan& pl, std::vector<std::array<int, 4>>& tabKeys, std::vector<int>& pt,
                    std::vector<int>& ps, std::string& msg) = 0;
   virtual int m2lLists(Plan& pl, std::string& msg) = 0;
+  // After m2lLists and the host's grouping of targets (m2lGroupTarget):
+  // fills m2lGroupBeg, m2lGSrcF, m2lGMask and m2lGCode as the host merge of
+  // a group's source rows does.
+  virtual int m2lGroups(Plan& pl, std::string& msg) = 0;
 };
 
 // Build the plan.  Returns an error code (OK on success); msg gets details.

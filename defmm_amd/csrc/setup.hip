@@ -305,6 +305,70 @@ __global__ __launch_bounds__(256) void k_list_place(ListArgs a) {
   }
 }
 
+// Grouped M2L lists: per group of up to G target slots, the union of their
+// source rows in ascending order, each with the mask of the targets reading
+// it and their (table << 6 | hash) codes (0 for targets not reading it).  A
+// thread per group merges its targets' lists, which ascend strictly (a
+// source row repeated within one target would make the host's grouping
+// differ: it is flagged and the plan goes to the host builder).
+struct GroupArgs {
+  int ngroups;
+  const int* target;  // ngroups x G target slots, -1 padded
+  const int* beg;     // M2L CSR by target slot
+  const int* srcF;
+  const int* tab;
+  const int* hash;
+  int* count;         // count pass: entries per group
+  const int* gBeg;    // write pass: first entry of each group
+  int* outSrc;
+  int* outMask;
+  int* outCode;       // G per entry
+  int* bad;
+};
+
+template <int G, bool WRITE>
+__global__ __launch_bounds__(256) void k_groups(GroupArgs a) {
+  const int gi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gi >= a.ngroups) return;
+  constexpr int kEnd = 0x7FFFFFFF;
+  int h[G], e[G], v[G];
+#pragma unroll
+  for (int g = 0; g < G; g++) {
+    const int t = a.target[(size_t)gi * G + g];
+    h[g] = t >= 0 ? a.beg[t] : 0;
+    e[g] = t >= 0 ? a.beg[t + 1] : 0;
+    v[g] = h[g] < e[g] ? a.srcF[h[g]] : kEnd;
+  }
+  int n = 0;
+  const int o = WRITE ? a.gBeg[gi] : 0;
+  for (;;) {
+    int m = kEnd;
+#pragma unroll
+    for (int g = 0; g < G; g++) m = min(m, v[g]);
+    if (m == kEnd) break;
+    int mask = 0;
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      int code = 0;
+      if (v[g] == m) {
+        mask |= 1 << g;
+        code = (a.tab[h[g]] << 6) | a.hash[h[g]];
+        h[g]++;
+        const int nv = h[g] < e[g] ? a.srcF[h[g]] : kEnd;
+        if (nv <= m) atomicOr(a.bad, 1);
+        v[g] = nv;
+      }
+      if (WRITE) a.outCode[(size_t)(o + n) * G + g] = code;
+    }
+    if (WRITE) {
+      a.outSrc[o + n] = m;
+      a.outMask[o + n] = mask;
+    }
+    n++;
+  }
+  if (!WRITE) a.count[gi] = n;
+}
+
 class GpuSetup final : public SetupDevice {
  public:
   ~GpuSetup() override {
@@ -313,8 +377,13 @@ class GpuSetup final : public SetupDevice {
   int walk(Plan& pl, std::vector<std::array<int, 4>>& tabKeys, std::vector<int>& pt,
            std::vector<int>& ps, std::string& msg) override;
   int m2lLists(Plan& pl, std::string& msg) override;
+  int m2lGroups(Plan& pl, std::string& msg) override;
 
  private:
+  // the M2L CSR entries left on the device by m2lLists
+  int* dSrcF_ = nullptr;
+  int* dTab_ = nullptr;
+  int* dHash_ = nullptr;
   struct Level {
     int l = 0, nt = 0, kind = 0, W = -1;
     int* tcell = nullptr;
@@ -696,6 +765,87 @@ int GpuSetup::m2lLists(Plan& pl, std::string& msg) {
   SCK(copyLarge(pl.m2lSrcF.data(), a.outSrcF, (size_t)nm * sizeof(int), false));
   SCK(copyLarge(pl.m2lTabId.data(), a.outTab, (size_t)nm * sizeof(int), false));
   SCK(copyLarge(pl.m2lHash.data(), a.outHash, (size_t)nm * sizeof(int), false));
+  dSrcF_ = a.outSrcF;
+  dTab_ = a.outTab;
+  dHash_ = a.outHash;
+  return OK;
+}
+
+template <int G>
+static void launchGroups(const GroupArgs& a, bool write) {
+  const unsigned nb = (unsigned)((a.ngroups + 255) / 256);
+  if (write) k_groups<G, true><<<nb, 256, 0, 0>>>(a);
+  else k_groups<G, false><<<nb, 256, 0, 0>>>(a);
+}
+
+int GpuSetup::m2lGroups(Plan& pl, std::string& msg) {
+  const int G = pl.m2lG;
+  const int64_t ng = (int64_t)pl.m2lGroupTarget.size() / G;
+  pl.m2lGroupBeg.assign(ng + 1, 0);
+  pl.m2lGSrcF.clear();
+  pl.m2lGMask.clear();
+  pl.m2lGCode.clear();
+  if (ng == 0) return OK;
+  if (G != 1 && G != 3 && G != 4) {
+    msg = "device set-up: unsupported M2L group size";
+    return kSetupFallback;
+  }
+  if (!pl.m2lSrcF.empty() && !dSrcF_) {
+    msg = "device set-up: M2L lists not on the device";
+    return kSetupFallback;
+  }
+  GroupArgs a = {};
+  a.ngroups = (int)ng;
+  int *tgt = nullptr, *beg = nullptr, *cnt = nullptr, *bad = nullptr;
+  SCK(put(tgt, pl.m2lGroupTarget));
+  SCK(put(beg, pl.m2lBeg));
+  SCK(alloc(cnt, (size_t)ng + 1));
+  SCK(alloc(bad, 1));
+  SCK(hipMemset(bad, 0, sizeof(int)));
+  a.target = tgt;
+  a.beg = beg;
+  a.srcF = dSrcF_;
+  a.tab = dTab_;
+  a.hash = dHash_;
+  a.count = cnt;
+  a.bad = bad;
+  auto launch = [&](bool write) {
+    if (G == 1) launchGroups<1>(a, write);
+    else if (G == 3) launchGroups<3>(a, write);
+    else launchGroups<4>(a, write);
+    return hipGetLastError();
+  };
+  SCK(launch(false));
+  std::vector<int> c(ng);
+  SCK(hipMemcpy(c.data(), cnt, (size_t)ng * sizeof(int), hipMemcpyDeviceToHost));
+  int flag = 0;
+  SCK(hipMemcpy(&flag, bad, sizeof(int), hipMemcpyDeviceToHost));
+  if (flag) {
+    msg = "device set-up: a target's M2L sources do not strictly ascend";
+    return kSetupFallback;
+  }
+  int64_t run = 0;
+  for (int64_t g = 0; g < ng; g++) {
+    run += c[g];
+    if (run >= ((int64_t)1 << 31)) {
+      msg = "M2L lists exceed 2^31 entries";
+      return ERR_MEMORY;
+    }
+    pl.m2lGroupBeg[g + 1] = (int)run;
+  }
+  SCK(hipMemcpy(cnt, pl.m2lGroupBeg.data(), ((size_t)ng + 1) * sizeof(int),
+                hipMemcpyHostToDevice));
+  a.gBeg = cnt;
+  SCK(alloc(a.outSrc, (size_t)run));
+  SCK(alloc(a.outMask, (size_t)run));
+  SCK(alloc(a.outCode, (size_t)run * G));
+  SCK(launch(true));
+  pl.m2lGSrcF.resize(run);
+  pl.m2lGMask.resize(run);
+  pl.m2lGCode.resize((size_t)run * G);
+  SCK(copyLarge(pl.m2lGSrcF.data(), a.outSrc, (size_t)run * sizeof(int), false));
+  SCK(copyLarge(pl.m2lGMask.data(), a.outMask, (size_t)run * sizeof(int), false));
+  SCK(copyLarge(pl.m2lGCode.data(), a.outCode, (size_t)run * G * sizeof(int), false));
   return OK;
 }
 
