@@ -36,6 +36,7 @@ def short(name):
     name = name.replace("HIP_vector_type<double, 2u>", "double2")
     name = name.replace("HIP_vector_type<float, 2u>", "float2")
     name = re.sub(r"^void ", "", name)
+    name = name.replace("(anonymous namespace)::", "")
     m = re.match(r"([\w:]+)(<[^(]*>)?\(", name)
     return (m.group(1) + (m.group(2) or "")) if m else name[:70]
 
@@ -201,12 +202,13 @@ def cmd_counters(a):
         per[k][r["Counter_Name"]] += float(r["Counter_Value"])
         calls[k].add(r["Dispatch_Id"])
     names = sorted({c for v in per.values() for c in v})
+    tanote = " (TA_BUSY_avr: averaged over TA instances, then summed)" if "TA_BUSY_avr" in names else ""
     out = [f"# Instruction counters per kernel: {a.title} (rocprofv3 --pmc)", "",
            "Command (MI355X, one GPU, repository root):",
            "  cd /tmp && export TMPDIR=/tmp && cd /root/repo"]
     out += ["  " + c for c in a.cmd]
     out += [f"  python3 profile_summary.py counters <dir> --title '{a.title}'", "",
-            "Sums over every dispatch of the run (TA_BUSY_avr: averaged over TA instances, then summed).  GRBM_GUI_ACTIVE is reported summed over the "
+            f"Sums over every dispatch of the run{tanote}.  GRBM_GUI_ACTIVE is reported summed over the "
             f"{a.xcds} XCDs, so kernel cycles = GRBM_GUI_ACTIVE / {a.xcds}.  MFMA busy = "
             f"SQ_VALU_MFMA_BUSY_CYCLES / (cycles x {a.simds} SIMDs) (rocprofv3's MfmaUtil); VALU "
             f"busy = 4 x SQ_INSTS_VALU / (cycles x {a.simds} SIMDs) (a wave64 VALU instruction "
