@@ -124,9 +124,10 @@ def _points(a):
 
 
 def set_setup_mode(mode):
-    """Where plan creation runs the dual walk and the M2L lists: SETUP_DEVICE
-    (the default, the current GPU) or SETUP_HOST (host C++).  Both build the
-    same plan.  Process-wide; returns the previous mode."""
+    """Where plan creation builds the octrees, runs the dual walk and builds
+    the M2L lists: SETUP_DEVICE (the default, the current GPU) or SETUP_HOST
+    (host C++).  Both build the same plan.  Process-wide; returns the
+    previous mode."""
     rc = lib().defmm_set_setup_mode(int(mode))
     if rc < 0:
         _check(-rc)

@@ -13,6 +13,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <numeric>
 #include <tuple>
@@ -141,6 +142,95 @@ static int buildTree(Tree& t, const double* xyz, int64_t n, const double ctr[3],
     t.x[i] = bd.px[p]; t.y[i] = bd.py[p]; t.z[i] = bd.pz[p];
   }
   return OK;
+}
+
+// Level-synchronous octree (the device builder's): per level the cells in
+// parent order and, within a parent, octant order; a splitting cell's points
+// are partitioned stably by octant, so every cell's points are a contiguous
+// range of the final point order, as in the recursive build above.
+bool cellStops(int64_t n, int lvl, int ncrit, int lvmax) {
+  return lvmax < 0 ? (n < ncrit) : (lvl >= lvmax);
+}
+
+// Children of a splitting cell from its octant counts, with the recursive
+// build's centre arithmetic (sort.hpp:98-162).
+void splitCell(LevelCell& c, const int64_t cnt[8], std::vector<LevelCell>& next) {
+  const double nr = c.rad / 2.;
+  int64_t off = c.b;
+  c.kid0 = (int)next.size();
+  c.nkid = 0;
+  for (int i = 0; i < 8; i++) {
+    if (!cnt[i]) continue;
+    LevelCell k;
+    k.b = off;
+    k.n = cnt[i];
+    for (int q = 0; q < 3; q++) {
+      k.ctr[q] = c.ctr[q] + nr * (((i >> q) & 1) * 2 - 1);
+      k.idx[q] = 2 * c.idx[q] + ((i >> q) & 1);
+    }
+    k.rad = nr;
+    next.push_back(k);
+    c.nkid++;
+    off += cnt[i];
+  }
+}
+
+// The recursive build's layout from a level-synchronous one: cells numbered
+// in pre-order, a cell's son list appended after its children's (as the
+// recursion appends it), points in the final order.
+void treeFromLevels(Tree& t, const std::vector<std::vector<LevelCell>>& lv, std::vector<int>& perm,
+                    const double* xyz) {
+  const int nl = (int)lv.size();
+  std::vector<std::vector<int>> size(nl), pre(nl);
+  for (int l = nl - 1; l >= 0; l--) {
+    size[l].assign(lv[l].size(), 1);
+    for (size_t c = 0; c < lv[l].size(); c++)
+      for (int k = 0; k < lv[l][c].nkid; k++) size[l][c] += size[l + 1][lv[l][c].kid0 + k];
+  }
+  for (int l = 0; l < nl; l++) pre[l].assign(lv[l].size(), 0);
+  for (int l = 0; l + 1 < nl; l++)
+    for (size_t c = 0; c < lv[l].size(); c++) {
+      int p = pre[l][c] + 1;
+      for (int k = 0; k < lv[l][c].nkid; k++) {
+        pre[l + 1][lv[l][c].kid0 + k] = p;
+        p += size[l + 1][lv[l][c].kid0 + k];
+      }
+    }
+  const int nc = size[0][0];
+  t.lvl.assign(nc, 0); t.nprt.assign(nc, 0); t.pbeg.assign(nc, 0);
+  t.son0.assign(nc, 0); t.nson.assign(nc, 0);
+  t.idx.assign(3 * (size_t)nc, 0); t.ctr.assign(3 * (size_t)nc, 0.); t.rad.assign(nc, 0.);
+  t.nPerLevel.assign(nl, 0);
+  for (int l = 0; l < nl; l++)
+    for (size_t c = 0; c < lv[l].size(); c++) {
+      const LevelCell& x = lv[l][c];
+      const int id = pre[l][c];
+      t.lvl[id] = l;
+      t.nprt[id] = (int)x.n;
+      t.pbeg[id] = (int)x.b;
+      t.nson[id] = x.nkid;
+      for (int q = 0; q < 3; q++) { t.idx[3 * id + q] = x.idx[q]; t.ctr[3 * id + q] = x.ctr[q]; }
+      t.rad[id] = x.rad;
+      t.nPerLevel[l]++;
+    }
+  t.sons.clear();
+  std::function<void(int, int)> post = [&](int l, int c) {
+    const LevelCell& x = lv[l][c];
+    for (int k = 0; k < x.nkid; k++) post(l + 1, x.kid0 + k);
+    if (x.nkid) {
+      t.son0[pre[l][c]] = (int)t.sons.size();
+      for (int k = 0; k < x.nkid; k++) t.sons.push_back(pre[l + 1][x.kid0 + k]);
+    }
+  };
+  post(0, 0);
+  const int64_t n = (int64_t)perm.size();
+  t.x.resize(n); t.y.resize(n); t.z.resize(n);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; i++) {
+    const int64_t p = perm[i];
+    t.x[i] = xyz[3 * p]; t.y[i] = xyz[3 * p + 1]; t.z[i] = xyz[3 * p + 2];
+  }
+  t.perm.swap(perm);
 }
 
 // --------------------------------------------------------- directions ----
@@ -601,8 +691,15 @@ int buildPlan(Plan& pl, const double* src, int64_t ns, const double* trg, int64_
     t0 = t1;
     return (int64_t)ms;
   };
-  int e = buildTree(pl.T, trg, nt, pl.boxCtr, pl.boxRad, ncrit, lvmax);
-  if (!e) e = buildTree(pl.S, src, ns, pl.boxCtr, pl.boxRad, ncrit, lvmax);
+  int e;
+  if (dev) {
+    e = dev->tree(pl.T, trg, nt, pl.boxCtr, pl.boxRad, ncrit, lvmax, msg);
+    if (!e) e = dev->tree(pl.S, src, ns, pl.boxCtr, pl.boxRad, ncrit, lvmax, msg);
+    if (e == kSetupFallback) return e;
+  } else {
+    e = buildTree(pl.T, trg, nt, pl.boxCtr, pl.boxRad, ncrit, lvmax);
+    if (!e) e = buildTree(pl.S, src, ns, pl.boxCtr, pl.boxRad, ncrit, lvmax);
+  }
   if (e) { msg = "octree deeper than 30 levels (too many coincident points for Ncrit)"; return e; }
 
   // frequency.hpp:43-129, 158-184 (eta = 1)

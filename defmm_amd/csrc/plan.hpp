@@ -149,7 +149,26 @@ struct Plan {
   Stats st;
 };
 
-// Set-up steps that a device can take over from the host builder.  walk()
+// A cell of a level-synchronous octree build: point range [b, b + n) of the
+// final point order, centre, half-width, integer coordinates, and its
+// children as a range of the next level's cells.
+struct LevelCell {
+  int64_t b = 0, n = 0;
+  double ctr[3] = {0, 0, 0}, rad = 0;
+  int idx[3] = {0, 0, 0};
+  int kid0 = 0, nkid = 0;
+};
+// Whether a cell of n points at level lvl is a leaf (sort.hpp:49-51).
+bool cellStops(int64_t n, int lvl, int ncrit, int lvmax);
+// Appends the children of a splitting cell (octant counts cnt) to next.
+void splitCell(LevelCell& c, const int64_t cnt[8], std::vector<LevelCell>& next);
+// Fills t (pre-order cells, son lists, points in tree order) from the
+// levels of a level-synchronous build and its final point order (consumed).
+void treeFromLevels(Tree& t, const std::vector<std::vector<LevelCell>>& lv, std::vector<int>& perm,
+                    const double* xyz);
+
+// Set-up steps that a device can take over from the host builder.  tree()
+// builds an octree where the host would.  walk()
 // runs where the host dual walk would: it sets the trees' effective-direction
 // marks and the walk statistics, returns the sorted table keys (level,
 // fundamental offset) and the P2P cell pairs, and keeps the M2L pairs.
@@ -162,6 +181,10 @@ struct Plan {
 constexpr int kSetupFallback = -1;
 struct SetupDevice {
   virtual ~SetupDevice() {}
+  // Builds one octree as the host builder does (same cells in pre-order,
+  // same point order, same centres): OK, ERR_DEPTH or kSetupFallback.
+  virtual int tree(Tree& t, const double* xyz, int64_t n, const double ctr[3], double rad,
+                   int ncrit, int lvmax, std::string& msg) = 0;
   virtual int walk(Plan& pl, std::vector<std::array<int, 4>>& tabKeys, std::vector<int>& pt,
                    std::vector<int>& ps, std::string& msg) = 0;
   virtual int m2lLists(Plan& pl, std::string& msg) = 0;

@@ -1,4 +1,11 @@
-// setup.hip -- the plan builder's dual walk and M2L lists on the GPU.
+// setup.hip -- the plan builder's octrees, dual walk and M2L lists on the GPU.
+//
+// The octrees are built level by level with the recursive build's rules
+// (sort.hpp:98-162 of the reference): each splitting cell's points are
+// partitioned stably by octant on the device (k_tree_count, k_tree_part),
+// the children and their centres are made on the host (splitCell), and the
+// cells are finally numbered in pre-order (treeFromLevels), which gives the
+// recursive build's tree exactly.
 //
 // The host builder (plan.cpp) walks the source and target octrees from the
 // roots (hrzpass.hpp:61-91 and the set-up walk hrzpass.hpp:111-138 of the
@@ -173,6 +180,85 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
     a.cM[t] = nM;
     a.cP[t] = nP;
     a.cE[t] = nE;
+  }
+}
+
+// Octree build, one level: the points of each splitting cell, in chunks of
+// kTreeChunk consecutive positions, a block per chunk and kTreePer points per
+// thread in position order.  k_tree_count writes each point's octant (the
+// host builder's test x_k > c_k) and the chunk's octant counts;
+// k_tree_part moves each point to its octant's range at the offset the host
+// derived from the counts plus its rank among the chunk's points of that
+// octant that precede it (a stable partition).
+constexpr int kTreePer = 16;
+constexpr int kTreeChunk = 256 * kTreePer;
+
+struct TreeChunk {
+  int cell;   // index among the level's splitting cells
+  int start;  // first position of the chunk
+  int len;
+};
+
+__global__ __launch_bounds__(256) void k_tree_count(const TreeChunk* __restrict__ ch,
+                                                    const double* __restrict__ cctr,
+                                                    const int* __restrict__ perm,
+                                                    const double* __restrict__ xyz,
+                                                    uint8_t* __restrict__ code, int* __restrict__ cnt) {
+  __shared__ int s[8];
+  const TreeChunk c = ch[blockIdx.x];
+  if (threadIdx.x < 8) s[threadIdx.x] = 0;
+  __syncthreads();
+  const double c0 = cctr[3 * c.cell], c1 = cctr[3 * c.cell + 1], c2 = cctr[3 * c.cell + 2];
+  int local[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int k = 0; k < kTreePer; k++) {
+    const int i = threadIdx.x * kTreePer + k;
+    if (i >= c.len) break;
+    const size_t p = (size_t)perm[c.start + i];
+    const int o = (xyz[3 * p] > c0 ? 1 : 0) | (xyz[3 * p + 1] > c1 ? 2 : 0) | (xyz[3 * p + 2] > c2 ? 4 : 0);
+    code[c.start + i] = (uint8_t)o;
+    local[o]++;
+  }
+#pragma unroll
+  for (int o = 0; o < 8; o++)
+    if (local[o]) atomicAdd(&s[o], local[o]);
+  __syncthreads();
+  if (threadIdx.x < 8) cnt[(size_t)blockIdx.x * 8 + threadIdx.x] = s[threadIdx.x];
+}
+
+__global__ __launch_bounds__(256) void k_tree_part(const TreeChunk* __restrict__ ch,
+                                                   const int* __restrict__ off,
+                                                   const uint8_t* __restrict__ code,
+                                                   const int* __restrict__ perm, int* __restrict__ out) {
+  __shared__ int sc[8][256];
+  const TreeChunk c = ch[blockIdx.x];
+  const int t = threadIdx.x;
+  int local[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int k = 0; k < kTreePer; k++) {
+    const int i = t * kTreePer + k;
+    if (i >= c.len) break;
+    local[code[c.start + i]]++;
+  }
+#pragma unroll
+  for (int o = 0; o < 8; o++) sc[o][t] = local[o];
+  __syncthreads();
+  if (t < 8) {
+    // exclusive prefix over the threads, one octant per thread
+    int run = 0;
+    for (int u = 0; u < 256; u++) {
+      const int v = sc[t][u];
+      sc[t][u] = run;
+      run += v;
+    }
+  }
+  __syncthreads();
+  int pos[8];
+#pragma unroll
+  for (int o = 0; o < 8; o++) pos[o] = off[(size_t)blockIdx.x * 8 + o] + sc[o][t];
+  for (int k = 0; k < kTreePer; k++) {
+    const int i = t * kTreePer + k;
+    if (i >= c.len) break;
+    const int o = code[c.start + i];
+    out[pos[o]++] = perm[c.start + i];
   }
 }
 
@@ -374,6 +460,8 @@ class GpuSetup final : public SetupDevice {
   ~GpuSetup() override {
     for (void* p : owned_) (void)hipFree(p);
   }
+  int tree(Tree& t, const double* xyz, int64_t n, const double ctr[3], double rad, int ncrit,
+           int lvmax, std::string& msg) override;
   int walk(Plan& pl, std::vector<std::array<int, 4>>& tabKeys, std::vector<int>& pt,
            std::vector<int>& ps, std::string& msg) override;
   int m2lLists(Plan& pl, std::string& msg) override;
@@ -441,6 +529,92 @@ class GpuSetup final : public SetupDevice {
   } while (0)
 
 inline unsigned blocksFor(int nt) { return (unsigned)((nt + kWaves - 1) / kWaves); }
+
+int GpuSetup::tree(Tree& t, const double* xyz, int64_t n, const double ctr[3], double rad,
+                   int ncrit, int lvmax, std::string& msg) {
+  double* dxyz = nullptr;
+  int *perm = nullptr, *perm2 = nullptr;
+  uint8_t* code = nullptr;
+  SCK(put(dxyz, xyz, 3 * (size_t)n));
+  {
+    std::vector<int> id(n);
+    for (int64_t i = 0; i < n; i++) id[i] = (int)i;
+    SCK(put(perm, id));
+  }
+  SCK(alloc(perm2, (size_t)n));
+  SCK(alloc(code, (size_t)n));
+  std::vector<std::vector<LevelCell>> lev(1);
+  LevelCell r;
+  r.b = 0;
+  r.n = n;
+  for (int q = 0; q < 3; q++) r.ctr[q] = ctr[q];
+  r.rad = rad;
+  lev[0].push_back(r);
+  for (int l = 0;; l++) {
+    std::vector<int> split;
+    for (int c = 0; c < (int)lev[l].size(); c++)
+      if (!cellStops(lev[l][c].n, l, ncrit, lvmax)) split.push_back(c);
+    if (split.empty()) break;
+    if (l >= 30) return ERR_DEPTH;  // the children would be deeper than 30 levels
+    std::vector<TreeChunk> chunks;
+    std::vector<double> cc(3 * split.size());
+    for (size_t s = 0; s < split.size(); s++) {
+      const LevelCell& c = lev[l][split[s]];
+      for (int q = 0; q < 3; q++) cc[3 * s + q] = c.ctr[q];
+      for (int64_t o = 0; o < c.n; o += kTreeChunk)
+        chunks.push_back({(int)s, (int)(c.b + o), (int)std::min<int64_t>(kTreeChunk, c.n - o)});
+    }
+    TreeChunk* dch = nullptr;
+    double* dcc = nullptr;
+    int *cnt = nullptr, *off = nullptr;
+    SCK(put(dch, chunks));
+    SCK(put(dcc, cc));
+    SCK(alloc(cnt, chunks.size() * 8));
+    k_tree_count<<<(unsigned)chunks.size(), 256, 0, 0>>>(dch, dcc, perm, dxyz, code, cnt);
+    SCK(hipGetLastError());
+    std::vector<int> hc(chunks.size() * 8);
+    SCK(hipMemcpy(hc.data(), cnt, hc.size() * sizeof(int), hipMemcpyDeviceToHost));
+    // octant totals per cell, then each chunk's first position per octant
+    std::vector<int64_t> tot(split.size() * 8, 0);
+    for (size_t k = 0; k < chunks.size(); k++)
+      for (int o = 0; o < 8; o++) tot[chunks[k].cell * 8 + o] += hc[k * 8 + o];
+    std::vector<int64_t> run(split.size() * 8);
+    for (size_t s = 0; s < split.size(); s++) {
+      int64_t b = lev[l][split[s]].b;
+      for (int o = 0; o < 8; o++) {
+        run[s * 8 + o] = b;
+        b += tot[s * 8 + o];
+      }
+    }
+    std::vector<int> ho(chunks.size() * 8);
+    for (size_t k = 0; k < chunks.size(); k++)
+      for (int o = 0; o < 8; o++) {
+        ho[k * 8 + o] = (int)run[chunks[k].cell * 8 + o];
+        run[chunks[k].cell * 8 + o] += hc[k * 8 + o];
+      }
+    SCK(put(off, ho));
+    // positions outside the splitting cells keep their points
+    SCK(hipMemcpy(perm2, perm, (size_t)n * sizeof(int), hipMemcpyDeviceToDevice));
+    k_tree_part<<<(unsigned)chunks.size(), 256, 0, 0>>>(dch, off, code, perm, perm2);
+    SCK(hipGetLastError());
+    std::swap(perm, perm2);
+    release(dch);
+    release(dcc);
+    release(cnt);
+    release(off);
+    std::vector<LevelCell> next;
+    for (size_t s = 0; s < split.size(); s++) splitCell(lev[l][split[s]], &tot[s * 8], next);
+    lev.push_back(std::move(next));
+  }
+  std::vector<int> hp(n);
+  SCK(copyLarge(hp.data(), perm, (size_t)n * sizeof(int), false));
+  release(dxyz);
+  release(perm);
+  release(perm2);
+  release(code);
+  treeFromLevels(t, lev, hp, xyz);
+  return OK;
+}
 
 int GpuSetup::walk(Plan& pl, std::vector<std::array<int, 4>>& tabKeys, std::vector<int>& pt,
                    std::vector<int>& ps, std::string& msg) {

@@ -63,9 +63,10 @@ int defmm_plan_create(const double* src_xyz, int64_t n_src, const double* trg_xy
                       int precision, defmm_plan* out);
 
 /*
- * Where defmm_plan_create runs the dual walk (the reference's set-up
+ * Where defmm_plan_create builds the octrees (the reference's
+ * include/structs/sort.hpp:98-162), runs the dual walk (its set-up
  * traversal, include/traversals/hrzpass.hpp:111-138, and its M2L list
- * building, include/interface.hpp:100-151) and the M2L lists:
+ * building, include/interface.hpp:100-151) and builds the M2L lists:
  * DEFMM_SETUP_DEVICE (the default) on the current GPU, DEFMM_SETUP_HOST in
  * host C++ (OpenMP).  Both build the same plan, entry for entry.  The device
  * path hands a plan it cannot take (an M2L offset window above 2^26

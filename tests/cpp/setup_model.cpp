@@ -3,6 +3,8 @@
 // array, with the plan the host builder makes for the same points.
 //
 // The model takes the device path's steps in the device path's order: the
+// level-synchronous octree build (a splitting cell's points partitioned
+// stably by octant, level by level, then renumbered in pre-order), the
 // level-synchronous walk (the sources a target meets at level l are the
 // children of the sources its parent expanded at level l - 1, visited in
 // the parent's list order and child order, as a wavefront's lanes do), the
@@ -70,6 +72,44 @@ struct ModelSetup final : SetupDevice {
     const int z0 = pl.T.idx[3 * i] - pl.S.idx[3 * j], z1 = pl.T.idx[3 * i + 1] - pl.S.idx[3 * j + 1],
               z2 = pl.T.idx[3 * i + 2] - pl.S.idx[3 * j + 2];
     return code[L.l][winIndex(z0, z1, z2, L.W)];
+  }
+
+  // level by level: each splitting cell's points partitioned stably by octant
+  int tree(Tree& t, const double* xyz, int64_t n, const double ctr[3], double rad, int ncrit,
+           int lvmax, std::string& msg) override {
+    std::vector<int> perm(n);
+    for (int64_t i = 0; i < n; i++) perm[i] = (int)i;
+    std::vector<std::vector<LevelCell>> lev(1);
+    LevelCell r;
+    r.b = 0;
+    r.n = n;
+    for (int q = 0; q < 3; q++) r.ctr[q] = ctr[q];
+    r.rad = rad;
+    lev[0].push_back(r);
+    for (int l = 0;; l++) {
+      std::vector<LevelCell> next;
+      for (LevelCell& c : lev[l]) {
+        if (cellStops(c.n, l, ncrit, lvmax)) continue;
+        if (l >= 30) return ERR_DEPTH;
+        int64_t cnt[8] = {0};
+        std::vector<int> bucket[8];
+        for (int64_t i = c.b; i < c.b + c.n; i++) {
+          const int64_t p = perm[i];
+          const int o = (xyz[3 * p] > c.ctr[0] ? 1 : 0) | (xyz[3 * p + 1] > c.ctr[1] ? 2 : 0) |
+                        (xyz[3 * p + 2] > c.ctr[2] ? 4 : 0);
+          bucket[o].push_back((int)p);
+          cnt[o]++;
+        }
+        int64_t w = c.b;
+        for (int o = 0; o < 8; o++)
+          for (int p : bucket[o]) perm[w++] = p;
+        splitCell(c, cnt, next);
+      }
+      if (next.empty()) break;
+      lev.push_back(std::move(next));
+    }
+    treeFromLevels(t, lev, perm, xyz);
+    return OK;
   }
 
   int walk(Plan& pl, std::vector<std::array<int, 4>>& tabKeys, std::vector<int>& pt,
@@ -277,6 +317,20 @@ std::string comparePlans(const Plan& A, const Plan& B) {
   for (int w = 0; w < 2; w++) {
     const Tree& a = w ? A.T : A.S;
     const Tree& b = w ? B.T : B.S;
+    same("perm", a.perm, b.perm, bad);
+    same("x", a.x, b.x, bad);
+    same("y", a.y, b.y, bad);
+    same("z", a.z, b.z, bad);
+    same("lvl", a.lvl, b.lvl, bad);
+    same("nprt", a.nprt, b.nprt, bad);
+    same("pbeg", a.pbeg, b.pbeg, bad);
+    same("son0", a.son0, b.son0, bad);
+    same("nson", a.nson, b.nson, bad);
+    same("sons", a.sons, b.sons, bad);
+    same("idx", a.idx, b.idx, bad);
+    same("ctr", a.ctr, b.ctr, bad);
+    same("rad", a.rad, b.rad, bad);
+    same("nPerLevel", a.nPerLevel, b.nPerLevel, bad);
     same("marks", a.marks, b.marks, bad);
     same("slot0", a.slot0, b.slot0, bad);
     same("nslot", a.nslot, b.nslot, bad);
