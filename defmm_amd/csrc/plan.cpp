@@ -1333,7 +1333,10 @@ int buildPlan(Plan& pl, const double* src, int64_t ns, const double* trg, int64_
   // ---- P2P, CSR by target leaf (P2P.hpp:42-79 over whole cell ranges) ----
   // A pair with a non-leaf target cell is split over its descendant leaves;
   // each leaf's source ranges are sorted and touching ranges merged.
-  {
+  if (dev) {
+    int r = dev->p2pLists(pl, msg);
+    if (r != OK) return r;
+  } else {
     const Tree& T = pl.T;
     std::vector<int> leafOfCell(T.ncells(), -1);
     std::vector<int> leaves;

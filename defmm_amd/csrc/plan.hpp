@@ -192,6 +192,10 @@ struct SetupDevice {
   // fills m2lGroupBeg, m2lGSrcF, m2lGMask and m2lGCode as the host merge of
   // a group's source rows does.
   virtual int m2lGroups(Plan& pl, std::string& msg) = 0;
+  // Where the host builds the P2P lists: fills p2pLeaf, p2pBeg, p2pSrcB and
+  // p2pSrcE from the P2P pairs the walk kept (walk() then returns no pairs
+  // to the host) and sets the P2P statistics.
+  virtual int p2pLists(Plan& pl, std::string& msg) = 0;
 };
 
 // Build the plan.  Returns an error code (OK on success); msg gets details.

@@ -1,4 +1,5 @@
-// setup.hip -- the plan builder's octrees, dual walk and M2L lists on the GPU.
+// setup.hip -- the plan builder's octrees, dual walk, M2L and P2P lists on
+// the GPU.
 //
 // The octrees are built level by level with the recursive build's rules
 // (sort.hpp:98-162 of the reference): each splitting cell's points are
@@ -455,6 +456,98 @@ __global__ __launch_bounds__(256) void k_groups(GroupArgs a) {
   if (!WRITE) a.count[gi] = n;
 }
 
+// P2P lists (P2P.hpp:42-79 over whole cell ranges): a target leaf's near
+// sources are those of the P2P pairs of the leaf and of each of its
+// ancestors (a pair with a non-leaf target covers all its leaves).  Each such
+// list is in ascending source cell order; a thread per leaf merges them by
+// source range (pbeg, pbeg + nprt), as the host sorts them, and joins
+// touching ranges, keeping others (overlapping ones included) separate.
+struct P2PArgs {
+  int nleaf;
+  const int* leaf;          // target leaves in pre-order
+  const int* parent;        // per target cell, -1 at the root
+  const int* lvl;
+  const int2* seg;          // per target cell: its P2P pairs [x, y) in its level's list
+  const int* const* src;    // per level: the P2P pairs' source cells
+  const int* Spbeg;
+  const int* Snprt;
+  const int* Tnprt;
+  int* count;               // count pass: ranges per leaf
+  unsigned long long* pairs;  // count pass: sum of n_target x n_source over the pairs
+  const int* beg;           // write pass: first range of each leaf
+  int* outB;
+  int* outE;
+};
+
+template <bool WRITE>
+__global__ __launch_bounds__(256) void k_p2p_leaf(P2PArgs a) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = k < a.nleaf;
+  unsigned long long pts = 0;
+  if (valid) {
+    const int leaf = a.leaf[k];
+    int hl[32], hp[32], he[32];
+    int nh = 0;
+    for (int c = leaf; c >= 0 && nh < 32; c = a.parent[c]) {
+      const int2 sg = a.seg[c];
+      if (sg.y > sg.x) {
+        hl[nh] = a.lvl[c];
+        hp[nh] = sg.x;
+        he[nh] = sg.y;
+        nh++;
+      }
+    }
+    int n = 0;
+    const int o = WRITE ? a.beg[k] : 0;
+    int cb = 0, ce = 0;
+    bool open = false;
+    for (;;) {
+      int best = -1, bb = 0x7FFFFFFF, be = 0x7FFFFFFF;
+      for (int h = 0; h < nh; h++) {
+        if (hp[h] >= he[h]) continue;
+        const int j = a.src[hl[h]][hp[h]];
+        const int rb = a.Spbeg[j], re = rb + a.Snprt[j];
+        if (rb < bb || (rb == bb && re < be)) {
+          best = h;
+          bb = rb;
+          be = re;
+        }
+      }
+      if (best < 0) break;
+      hp[best]++;
+      pts += (unsigned long long)(be - bb);
+      if (open && bb == ce) {
+        ce = be;
+      } else {
+        if (open) {
+          if (WRITE) {
+            a.outB[o + n] = cb;
+            a.outE[o + n] = ce;
+          }
+          n++;
+        }
+        cb = bb;
+        ce = be;
+        open = true;
+      }
+    }
+    if (open) {
+      if (WRITE) {
+        a.outB[o + n] = cb;
+        a.outE[o + n] = ce;
+      }
+      n++;
+    }
+    if (!WRITE) a.count[k] = n;
+    pts *= (unsigned long long)a.Tnprt[leaf];
+  }
+  if (!WRITE) {
+    // one atomic per wavefront
+    for (int off = 32; off > 0; off >>= 1) pts += __shfl_down(pts, off);
+    if ((threadIdx.x & 63) == 0 && pts) atomicAdd(a.pairs, pts);
+  }
+}
+
 class GpuSetup final : public SetupDevice {
  public:
   ~GpuSetup() override {
@@ -466,8 +559,13 @@ class GpuSetup final : public SetupDevice {
            std::vector<int>& ps, std::string& msg) override;
   int m2lLists(Plan& pl, std::string& msg) override;
   int m2lGroups(Plan& pl, std::string& msg) override;
+  int p2pLists(Plan& pl, std::string& msg) override;
 
  private:
+  // P2P pairs the walk found: per level the source cells, per target cell
+  // its range in its level's list
+  std::vector<int*> pLv_;
+  std::vector<int2> pSeg_;
   // the M2L CSR entries left on the device by m2lLists
   int* dSrcF_ = nullptr;
   int* dTab_ = nullptr;
@@ -631,12 +729,14 @@ int GpuSetup::walk(Plan& pl, std::vector<std::array<int, 4>>& tabKeys, std::vect
   ps.clear();
   int64_t nHF = 0, nLF = 0, nP2P = 0;
   // the root pair (level 0 has no M2L)
+  pLv_.assign(pl.nlvl, nullptr);
+  pSeg_.assign(T.ncells(), make_int2(0, 0));
   if (T.leaf(0) || S.leaf(0)) {
-    pt.push_back(0);
-    ps.push_back(0);
+    const int root = 0;
+    SCK(put(pLv_[0], &root, 1));
+    pSeg_[0] = make_int2(0, 1);
     pl.st.hf_m2l = pl.st.lf_m2l = 0;
     pl.st.p2p_calls = 1;
-    pl.st.p2p_pairs = (int64_t)T.nprt[0] * S.nprt[0];
     tabKeys.clear();
     return OK;
   }
@@ -743,17 +843,10 @@ int GpuSetup::walk(Plan& pl, std::vector<std::array<int, 4>>& tabKeys, std::vect
     a.eOut = eOut;
     k_walk<true><<<blocksFor(nt), 256, 0, 0>>>(a);
     SCK(hipGetLastError());
-    // P2P pairs to the host, target cells from the segments
-    if (tot[1]) {
-      const size_t p0 = ps.size();
-      ps.resize(p0 + (size_t)tot[1]);
-      SCK(copyLarge(ps.data() + p0, pSrc, (size_t)tot[1] * sizeof(int), false));
-      pt.resize(ps.size());
-      for (int t = 0; t < nt; t++)
-        for (int q = off[(size_t)(nt + 1) + t]; q < off[(size_t)(nt + 1) + t + 1]; q++)
-          pt[p0 + q] = cells[l][t];
-    }
-    release(pSrc);
+    // P2P pairs stay on the device for the P2P lists
+    pLv_[l] = pSrc;
+    for (int t = 0; t < nt; t++)
+      pSeg_[cells[l][t]] = make_int2(off[(size_t)(nt + 1) + t], off[(size_t)(nt + 1) + t + 1]);
     release(tparD);
     // M2L segments: keep the offsets on both sides
     SCK(alloc(L.mBeg, (size_t)nt + 1));
@@ -836,13 +929,9 @@ int GpuSetup::walk(Plan& pl, std::vector<std::array<int, 4>>& tabKeys, std::vect
   release(Snson);
   release(Sson0);
   release(Ssons);
-  int64_t pairs = 0;
-  const int64_t np = (int64_t)pt.size();
-  for (int64_t k = 0; k < np; k++) pairs += (int64_t)T.nprt[pt[k]] * S.nprt[ps[k]];
   pl.st.hf_m2l = nHF;
   pl.st.lf_m2l = nLF;
   pl.st.p2p_calls = nP2P;
-  pl.st.p2p_pairs = pairs;
   return OK;
 }
 
@@ -1020,6 +1109,86 @@ int GpuSetup::m2lGroups(Plan& pl, std::string& msg) {
   SCK(copyLarge(pl.m2lGSrcF.data(), a.outSrc, (size_t)run * sizeof(int), false));
   SCK(copyLarge(pl.m2lGMask.data(), a.outMask, (size_t)run * sizeof(int), false));
   SCK(copyLarge(pl.m2lGCode.data(), a.outCode, (size_t)run * G * sizeof(int), false));
+  return OK;
+}
+
+int GpuSetup::p2pLists(Plan& pl, std::string& msg) {
+  const Tree &T = pl.T, &S = pl.S;
+  pl.p2pLeaf.clear();
+  pl.p2pBeg.assign(1, 0);
+  pl.p2pSrcB.clear();
+  pl.p2pSrcE.clear();
+  std::vector<int> leaves, parent(T.ncells(), -1);
+  for (int c = 0; c < T.ncells(); c++) {
+    if (T.leaf(c)) leaves.push_back(c);
+    for (int k = 0; k < T.nson[c]; k++) parent[T.sons[T.son0[c] + k]] = c;
+  }
+  const int nl = (int)leaves.size();
+  if ((int)pLv_.size() != pl.nlvl || (int)pSeg_.size() != T.ncells()) {
+    msg = "device set-up: P2P pairs not on the device";
+    return kSetupFallback;
+  }
+  P2PArgs a = {};
+  a.nleaf = nl;
+  int *dl = nullptr, *dp = nullptr, *dlv = nullptr, *dsb = nullptr, *dsn = nullptr, *dtn = nullptr;
+  int2* dseg = nullptr;
+  int** dsrc = nullptr;
+  int* cnt = nullptr;
+  unsigned long long* pairs = nullptr;
+  SCK(put(dl, leaves));
+  SCK(put(dp, parent));
+  SCK(put(dlv, T.lvl));
+  SCK(put(dseg, pSeg_));
+  SCK(put(dsrc, pLv_));
+  SCK(put(dsb, S.pbeg));
+  SCK(put(dsn, S.nprt));
+  SCK(put(dtn, T.nprt));
+  SCK(alloc(cnt, (size_t)nl + 1));
+  SCK(alloc(pairs, 1));
+  SCK(hipMemset(pairs, 0, sizeof(unsigned long long)));
+  a.leaf = dl;
+  a.parent = dp;
+  a.lvl = dlv;
+  a.seg = dseg;
+  a.src = dsrc;
+  a.Spbeg = dsb;
+  a.Snprt = dsn;
+  a.Tnprt = dtn;
+  a.count = cnt;
+  a.pairs = pairs;
+  const unsigned nb = (unsigned)((nl + 255) / 256);
+  k_p2p_leaf<false><<<nb, 256, 0, 0>>>(a);
+  SCK(hipGetLastError());
+  std::vector<int> c(nl);
+  SCK(hipMemcpy(c.data(), cnt, (size_t)nl * sizeof(int), hipMemcpyDeviceToHost));
+  unsigned long long np = 0;
+  SCK(hipMemcpy(&np, pairs, sizeof(np), hipMemcpyDeviceToHost));
+  pl.st.p2p_pairs = (int64_t)np;
+  std::vector<int> beg((size_t)nl + 1, 0);
+  int64_t run = 0;
+  for (int k = 0; k < nl; k++) {
+    beg[k] = (int)run;
+    run += c[k];
+    if (run >= ((int64_t)1 << 31)) {
+      msg = "P2P lists exceed 2^31 ranges";
+      return ERR_MEMORY;
+    }
+    if (c[k]) {
+      pl.p2pLeaf.push_back(leaves[k]);
+      pl.p2pBeg.push_back((int)run);
+    }
+  }
+  beg[nl] = (int)run;
+  SCK(hipMemcpy(cnt, beg.data(), ((size_t)nl + 1) * sizeof(int), hipMemcpyHostToDevice));
+  a.beg = cnt;
+  SCK(alloc(a.outB, (size_t)run));
+  SCK(alloc(a.outE, (size_t)run));
+  k_p2p_leaf<true><<<nb, 256, 0, 0>>>(a);
+  SCK(hipGetLastError());
+  pl.p2pSrcB.resize(run);
+  pl.p2pSrcE.resize(run);
+  SCK(copyLarge(pl.p2pSrcB.data(), a.outB, (size_t)run * sizeof(int), false));
+  SCK(copyLarge(pl.p2pSrcE.data(), a.outE, (size_t)run * sizeof(int), false));
   return OK;
 }
 

@@ -11,7 +11,8 @@
 // smallest (target, source) pair as each offset's representative, the
 // host's offsetCodes for table ids, hashes and cones, the marks of both
 // cells of each HF pair, the slot counts, the stable placement of each
-// target cell's pairs, and the merge of a group's source rows.  What it
+// target cell's pairs, the merge of a group's source rows, and per target
+// leaf the merge of its and its ancestors' P2P source lists.  What it
 // checks without a GPU is that these steps, and the host functions they
 // use (m2lWindow, offsetCodes), give the host builder's plan: the claims
 // the device path rests on (one target's sources met in ascending cell
@@ -67,6 +68,10 @@ struct ModelSetup final : SetupDevice {
   std::vector<Level> lv;
   std::vector<std::vector<uint64_t>> code;
   std::vector<int> srcF, tab, hash;
+  // P2P pairs kept by the walk: per level the source cells, per target cell
+  // its range [first, second) in its level's list
+  std::vector<std::vector<int>> pLv;
+  std::vector<std::pair<int, int>> pSeg;
 
   uint64_t codeOf(const Plan& pl, const Level& L, int i, int j) const {
     const int z0 = pl.T.idx[3 * i] - pl.S.idx[3 * j], z1 = pl.T.idx[3 * i + 1] - pl.S.idx[3 * j + 1],
@@ -122,12 +127,13 @@ struct ModelSetup final : SetupDevice {
     }
     pt.clear();
     ps.clear();
+    pLv.assign(pl.nlvl, {});
+    pSeg.assign(T.ncells(), {0, 0});
     if (T.leaf(0) || S.leaf(0)) {
-      pt.push_back(0);
-      ps.push_back(0);
+      pLv[0] = {0};
+      pSeg[0] = {0, 1};
       pl.st.hf_m2l = pl.st.lf_m2l = 0;
       pl.st.p2p_calls = 1;
-      pl.st.p2p_pairs = (int64_t)T.nprt[0] * S.nprt[0];
       tabKeys.clear();
       return OK;
     }
@@ -155,6 +161,7 @@ struct ModelSetup final : SetupDevice {
       std::vector<int> nb = {0}, ns;
       for (int i : cells[l]) {
         const int p = pos[parent[i]];
+        pSeg[i].first = (int)pLv[l].size();
         for (int e = eBeg[p]; e < eBeg[p + 1]; e++) {
           const int q = eSrc[e];
           for (int b = 0; b < S.nson[q]; b++) {
@@ -171,8 +178,7 @@ struct ModelSetup final : SetupDevice {
               const uint64_t v = ((uint64_t)(uint32_t)i << 32) | (uint32_t)j;
               if (v < r) r = v;
             } else if (T.leaf(i) || S.leaf(j)) {
-              pt.push_back(i);
-              ps.push_back(j);
+              pLv[l].push_back(j);
               nP++;
             } else {
               ns.push_back(j);
@@ -181,6 +187,7 @@ struct ModelSetup final : SetupDevice {
         }
         L.mBeg.push_back((int)L.mSrc.size());
         nb.push_back((int)ns.size());
+        pSeg[i].second = (int)pLv[l].size();
       }
       if (L.kind == 1) nHF += (int64_t)L.mSrc.size();
       if (L.kind == 2) nLF += (int64_t)L.mSrc.size();
@@ -202,11 +209,57 @@ struct ModelSetup final : SetupDevice {
         }
       }
     }
-    int64_t pairs = 0;
-    for (size_t k = 0; k < pt.size(); k++) pairs += (int64_t)T.nprt[pt[k]] * S.nprt[ps[k]];
     pl.st.hf_m2l = nHF;
     pl.st.lf_m2l = nLF;
     pl.st.p2p_calls = nP;
+    return OK;
+  }
+
+  // per target leaf: its and its ancestors' P2P source lists merged by
+  // source range, touching ranges joined
+  int p2pLists(Plan& pl, std::string& msg) override {
+    const Tree &T = pl.T, &S = pl.S;
+    std::vector<int> parent(T.ncells(), -1);
+    for (int c = 0; c < T.ncells(); c++)
+      for (int k = 0; k < T.nson[c]; k++) parent[T.sons[T.son0[c] + k]] = c;
+    pl.p2pLeaf.clear();
+    pl.p2pBeg.assign(1, 0);
+    pl.p2pSrcB.clear();
+    pl.p2pSrcE.clear();
+    int64_t pairs = 0;
+    for (int leaf = 0; leaf < T.ncells(); leaf++) {
+      if (!T.leaf(leaf)) continue;
+      std::vector<std::array<int, 3>> heads;  // level, next, end
+      for (int c = leaf; c >= 0; c = parent[c])
+        if (pSeg[c].second > pSeg[c].first) heads.push_back({T.lvl[c], pSeg[c].first, pSeg[c].second});
+      int cb = 0, ce = 0, n = 0;
+      bool open = false;
+      for (;;) {
+        int best = -1, bb = 0x7FFFFFFF, be = 0x7FFFFFFF;
+        for (size_t h = 0; h < heads.size(); h++) {
+          if (heads[h][1] >= heads[h][2]) continue;
+          const int j = pLv[heads[h][0]][heads[h][1]];
+          const int rb = S.pbeg[j], re = rb + S.nprt[j];
+          if (rb < bb || (rb == bb && re < be)) { best = (int)h; bb = rb; be = re; }
+        }
+        if (best < 0) break;
+        heads[best][1]++;
+        pairs += (int64_t)(be - bb) * T.nprt[leaf];
+        if (open && bb == ce) {
+          ce = be;
+        } else {
+          if (open) { pl.p2pSrcB.push_back(cb); pl.p2pSrcE.push_back(ce); n++; }
+          cb = bb;
+          ce = be;
+          open = true;
+        }
+      }
+      if (open) { pl.p2pSrcB.push_back(cb); pl.p2pSrcE.push_back(ce); n++; }
+      if (n) {
+        pl.p2pLeaf.push_back(leaf);
+        pl.p2pBeg.push_back((int)pl.p2pSrcB.size());
+      }
+    }
     pl.st.p2p_pairs = pairs;
     return OK;
   }

@@ -1,9 +1,10 @@
 """The device set-up's algorithm (defmm_amd/csrc/setup.hip) on the CPU: a
-C++ model of its steps (level-synchronous walk in the wavefronts' order,
-smallest pair per offset, offsetCodes, marks, stable per-slot placement,
-group merge) plugged into the plan builder's SetupDevice hook must give the
-host builder's plan array for array (marks, slots, tables, M2L CSR and
-groups, P2P ranges, M2M/L2L lists, statistics) on eight workloads: HF and
+C++ model of its steps (level-synchronous octree, level-synchronous walk in
+the wavefronts' order, smallest pair per offset, offsetCodes, marks, stable
+per-slot placement, group merge, per-leaf P2P merge) plugged into the plan
+builder's SetupDevice hook must give the host builder's plan array for array
+(trees, marks, slots, tables, M2L CSR and groups, P2P ranges, M2M/L2L lists,
+statistics) on eight workloads: HF and
 LF levels, Laplace, fixed depth, distinct clouds, order 8, a leaf target
 root.  This pins, without a GPU, the host functions the device path uses
 (m2lWindow, offsetCodes) and the order claims it rests on."""
