@@ -2569,9 +2569,9 @@ static int upload(Exec& X, Plan& pl) {
       d.hf = tl.hf;
       CKR(X.upv(d.out, tl.out)); CKR(X.upv(d.beg, tl.beg)); CKR(X.upv(d.in, tl.in));
       CKR(X.upv(d.sgn, tl.sgn)); CKR(X.upv(d.sigIdx, tl.sigIdx));
-      std::vector<double2> e(tl.edv.size());
-      for (size_t k = 0; k < e.size(); k++) e[k] = make_double2(tl.edv[k].real(), tl.edv[k].imag());
-      CKR(X.upv(d.edv, e));
+      // std::complex<double> is laid out as two doubles (real, imaginary)
+      static_assert(sizeof(cplx) == sizeof(double2), "complex layout");
+      CKR(X.up(d.edv, reinterpret_cast<const double2*>(tl.edv.data()), tl.edv.size()));
       dst.push_back(d);
     }
   }
