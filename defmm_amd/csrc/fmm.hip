@@ -2644,8 +2644,9 @@ static int upload(Exec& X, Plan& pl) {
   X.hM2fSlot = pl.m2fSlot;
   X.hGroupBeg = pl.m2lGroupBeg;
   {
-    std::vector<int> th(pl.m2lTabId.size());
-    const size_t n = th.size();
+    // every element is written below: no zero fill of the 4 bytes per entry
+    const size_t n = pl.m2lTabId.size();
+    std::unique_ptr<int[]> th(new int[std::max<size_t>(n, 1)]);
     const unsigned nth = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
     std::vector<std::thread> ts;
     for (unsigned t = 0; t < nth; t++)
@@ -2654,7 +2655,7 @@ static int upload(Exec& X, Plan& pl) {
           th[k] = (pl.m2lTabId[k] << 6) | pl.m2lHash[k];
       });
     for (auto& x : ts) x.join();
-    CKR(X.upv(X.m2lTabHash, th));
+    CKR(X.up(X.m2lTabHash, n ? th.get() : nullptr, n));
   }
   // P2P work items: 64-target chunks x runs of <= budget sources.  The
   // budget is 4,096 sources unless that leaves fewer than 16,384 items (too
