@@ -35,7 +35,8 @@
 // per slot (atomics), the host's prefix sum, then a stable placement in
 // which the wavefront of a target cell places its pairs in segment order,
 // one distinct target slot at a time (a target slot belongs to one cell,
-// so its cursor has one writer).
+// so its cursor has one writer).  The grouped M2L lists and the P2P lists
+// are merges of lists that are already in order (k_groups, k_p2p_leaf).
 //
 // Every decision is computed as on the host: the HF test in double
 // precision with the same operations in the same order and no contraction
