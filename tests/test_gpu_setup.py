@@ -1,11 +1,12 @@
-"""The plan builder's dual walk and M2L lists on the GPU (setup.hip) against
-the host builder (plan.cpp): for the same points and parameters both give
-the same plan -- every list statistic equal (pair counts, slots, tables,
-M2F rows, M2L entries and groups, P2P items, table-store rows) and
-executions bit-identical, which needs the M2L entries of every target slot
-in the same order.  Workloads cover HF and LF levels, Laplace (kappa = 0),
-a fixed-depth tree, different source and target clouds, and a tree too
-shallow for any M2L level (both builders refuse it alike)."""
+"""The plan builder's octrees, dual walk, M2L and P2P lists on the GPU
+(setup.hip) against the host builder (plan.cpp): for the same points and
+parameters both give the same plan -- every list statistic equal (pair
+counts, slots, tables, M2F rows, M2L entries and groups, P2P items,
+table-store rows) and executions bit-identical, which needs the M2L entries
+of every target slot in the same order.  Workloads cover HF and LF levels,
+a 200,000-point sphere at 64 wavelengths, Laplace (kappa = 0), a
+fixed-depth tree, different source and target clouds, a leaf target root,
+and a tree too shallow for any M2L level (both builders refuse it alike)."""
 import sys
 
 import numpy as np
@@ -82,6 +83,15 @@ def test_hf_workloads(geo, n, nl, L, ncrit):
            "cube_edges": lambda: wl.cube_edges(n, seed=33)}[geo]()
     sd = _compare(f"{geo} {nl} wavelengths", pts, pts, L, ncrit, wl.kappa_for(nl, pts))
     assert sd["hf_m2l"] > 0
+
+
+def test_sphere_200k_64_wavelengths():
+    """A larger plan in the benchmark's all-HF regime (200,000 points on the
+    sphere, 64 wavelengths, order 5, leaf <= 64: about 20M M2L entries)."""
+    from defmm_amd import workloads as wl
+    pts = wl.sphere(200000, seed=43)
+    sd = _compare("sphere 200k, 64 wavelengths", pts, pts, 5, 65, wl.kappa_for(64, pts))
+    assert sd["hf_m2l"] > 10_000_000
 
 
 def test_mixed_precision_several_rhs():
