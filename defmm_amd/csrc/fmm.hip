@@ -2074,7 +2074,8 @@ struct Exec {
   std::vector<int> hBatchGroupBeg, hBatchRowBeg;
   // host copies of the grouped entries' global F rows, the F-row -> source
   // slot map and the group bounds (the mode-dependent lists are built from them)
-  std::vector<int> hGSrcF, hM2fSlot, hGroupBeg;
+  ListVec hGSrcF;
+  std::vector<int> hM2fSlot, hGroupBeg;
   double2 *qio = nullptr, *pio = nullptr;
   double stageMs[DEFMM_NSTAGES] = {};
   // host copies needed to re-upload the typed buffers in another precision
@@ -2154,8 +2155,8 @@ struct Exec {
     if (src) HIPCK(copyLarge(d, src, n * sizeof(X), true));
     return OK;
   }
-  template <class X>
-  int upv(X*& dst, const std::vector<X>& v) {
+  template <class X, class A>
+  int upv(X*& dst, const std::vector<X, A>& v) {
     return up(dst, v.empty() ? nullptr : v.data(), v.size());
   }
   int alloc(void*& dst, size_t nbytes) {

@@ -10,11 +10,38 @@
 #include <cstdint>
 #include <map>
 #include <string>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 namespace dfm {
 
 typedef std::complex<double> cplx;
+
+// Allocator whose construction without a value leaves the element
+// default-initialised (an int: uninitialised), so that resize() of the large
+// M2L lists does not zero-fill memory that is overwritten right after (the
+// device set-up copies them back into host vectors of hundreds of millions
+// of entries).
+template <class T>
+struct NoInitAlloc : std::allocator<T> {
+  template <class U>
+  struct rebind {
+    typedef NoInitAlloc<U> other;
+  };
+  NoInitAlloc() = default;
+  template <class U>
+  NoInitAlloc(const NoInitAlloc<U>&) noexcept {}
+  template <class U>
+  void construct(U* p) noexcept(std::is_nothrow_default_constructible<U>::value) {
+    ::new ((void*)p) U;
+  }
+  template <class U, class... A>
+  void construct(U* p, A&&... a) {
+    ::new ((void*)p) U(std::forward<A>(a)...);
+  }
+};
+typedef std::vector<int, NoInitAlloc<int>> ListVec;
 
 // Target slots per M2L work group: a thread holds one accumulator per group
 // target and frequency it owns (2 targets at L = 8, where a thread owns 4
@@ -131,16 +158,17 @@ struct Plan {
   std::vector<int> m2fIndex;    // per source slot: row in the F store, -1 if none
   // M2L, CSR by target slot (every target slot appears, possibly empty)
   std::vector<int> m2lBeg;      // nslots(T)+1
-  std::vector<int> m2lSrcF;     // F row of the source slot
-  std::vector<int> m2lTabId;    // kernel table
-  std::vector<int> m2lHash;     // permutation
+  ListVec m2lSrcF;              // F row of the source slot
+  ListVec m2lTabId;             // kernel table
+  ListVec m2lHash;              // permutation
   std::vector<int> m2lOrder;    // target slots with M2L entries, in processing order
   std::vector<int> m2lZero;     // target slots without M2L entries (L = 0)
   // grouped M2L: per group m2lG target slots (-1 padded); per entry a source
   // F row, the mask of the group's targets reading it and their codes
   // (table id << 6 | permutation hash), m2lG per entry
   int m2lG = 4;
-  std::vector<int> m2lGroupBeg, m2lGroupTarget, m2lGSrcF, m2lGMask, m2lGCode;
+  std::vector<int> m2lGroupBeg, m2lGroupTarget;
+  ListVec m2lGSrcF, m2lGMask, m2lGCode;
   // P2P, CSR by target leaf: source particle ranges [b, e)
   std::vector<int> p2pLeaf;     // target leaf cells that receive near field
   std::vector<int> p2pBeg;      // CSR

@@ -67,7 +67,7 @@ struct ModelSetup final : SetupDevice {
   };
   std::vector<Level> lv;
   std::vector<std::vector<uint64_t>> code;
-  std::vector<int> srcF, tab, hash;
+  ListVec srcF, tab, hash;
   // P2P pairs kept by the walk: per level the source cells, per target cell
   // its range [first, second) in its level's list
   std::vector<std::vector<int>> pLv;
