@@ -67,6 +67,13 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise DefmmError(f"{LIB_PATH} is not built; run python defmm_amd/build.py")
+    # PyTorch ships its own HIP runtime.  Loaded first, it is the one the
+    # library binds to, so that both share one runtime; loaded after the
+    # library (which would then bring the system's), torch finds no GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     D = ctypes.POINTER(ctypes.c_double)
     I64 = ctypes.POINTER(ctypes.c_int64)

@@ -2631,11 +2631,11 @@ static int upload(Exec& X, Plan& pl) {
       }
     }
     std::vector<std::vector<uint8_t>>().swap(used);
-    std::vector<int> dc(pl.m2lGCode.size(), 0);
+    ListVec dc(pl.m2lGCode.size());  // every element written below
     parallel([&](size_t e0, size_t e1, unsigned) {
       for (size_t e = e0; e < e1; e++)
         for (int g = 0; g < G; g++)
-          if ((pl.m2lGMask[e] >> g) & 1) dc[e * G + g] = rowOf[keyOf(pl.m2lGCode[e * G + g])] << 3;
+          dc[e * G + g] = ((pl.m2lGMask[e] >> g) & 1) ? rowOf[keyOf(pl.m2lGCode[e * G + g])] << 3 : 0;
     });
     if (X.hRowKey.empty()) X.hRowKey.push_back(0);
     X.st.m2l_table_rows = (int64_t)X.hRowKey.size();
