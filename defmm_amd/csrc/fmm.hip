@@ -2045,8 +2045,7 @@ struct Exec {
   std::vector<DevTrans> m2m, l2l;
   int *m2fSlot = nullptr;
   int nm2f = 0;
-  int *m2lBeg = nullptr, *m2lSrcF = nullptr, *m2lTabHash = nullptr;
-  int *m2lOrder = nullptr, *m2lZero = nullptr;
+  int *m2lZero = nullptr;
   int nM2LTargets = 0, nM2LZero = 0;
   int *gBeg = nullptr, *gTarget = nullptr, *gMask = nullptr, *gCode = nullptr;
   // per group entry, the row it reads: a global F row (stored), a source
@@ -2577,8 +2576,9 @@ static int upload(Exec& X, Plan& pl) {
     }
   }
   CKR(X.upv(X.m2fSlot, pl.m2fSlot)); X.nm2f = (int)pl.m2fSlot.size();
-  CKR(X.upv(X.m2lBeg, pl.m2lBeg)); CKR(X.upv(X.m2lSrcF, pl.m2lSrcF));
-  CKR(X.upv(X.m2lOrder, pl.m2lOrder)); X.nM2LTargets = (int)pl.m2lOrder.size();
+  // the M2L kernels read the grouped lists; the per-target CSR and the
+  // processing order stay on the host
+  X.nM2LTargets = (int)pl.m2lOrder.size();
   CKR(X.upv(X.m2lZero, pl.m2lZero)); X.nM2LZero = (int)pl.m2lZero.size();
   CKR(X.upv(X.gBeg, pl.m2lGroupBeg)); X.nGroups = (int)pl.m2lGroupBeg.size() - 1;
   X.m2lG = pl.m2lG;
@@ -2644,20 +2644,6 @@ static int upload(Exec& X, Plan& pl) {
   X.hGSrcF.swap(pl.m2lGSrcF);
   X.hM2fSlot = pl.m2fSlot;
   X.hGroupBeg = pl.m2lGroupBeg;
-  {
-    // every element is written below: no zero fill of the 4 bytes per entry
-    const size_t n = pl.m2lTabId.size();
-    std::unique_ptr<int[]> th(new int[std::max<size_t>(n, 1)]);
-    const unsigned nth = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
-    std::vector<std::thread> ts;
-    for (unsigned t = 0; t < nth; t++)
-      ts.emplace_back([&, t] {
-        for (size_t k = n * t / nth; k < n * (t + 1) / nth; k++)
-          th[k] = (pl.m2lTabId[k] << 6) | pl.m2lHash[k];
-      });
-    for (auto& x : ts) x.join();
-    CKR(X.up(X.m2lTabHash, n ? th.get() : nullptr, n));
-  }
   // P2P work items: 64-target chunks x runs of <= budget sources.  The
   // budget is 4,096 sources unless that leaves fewer than 16,384 items (too
   // few waves to fill 1,024 SIMDs: the 20,000-point cube has 3,249); then it

@@ -1135,6 +1135,7 @@ int buildPlan(Plan& pl, const double* src, int64_t ns, const double* trg, int64_
     // counting sort over contiguous chunks of the entries, one chunk per thread
     pl.m2lBeg.assign(pl.T.nslots + 1, 0);
     pl.m2lSrcF.resize(nm); pl.m2lTabId.resize(nm); pl.m2lHash.resize(nm);
+    pl.m2lEntries = nm;
     {
       int nth = 1;
 #ifdef _OPENMP
@@ -1177,7 +1178,7 @@ int buildPlan(Plan& pl, const double* src, int64_t ns, const double* trg, int64_
     }
   }
   pl.st.m2f_slots = (int64_t)pl.m2fSlot.size();
-  pl.st.m2l_entries = (int64_t)pl.m2lSrcF.size();
+  pl.st.m2l_entries = pl.m2lEntries;
   for (int s = 0; s < pl.T.nslots; s++)
     if (pl.m2lBeg[s + 1] > pl.m2lBeg[s]) pl.st.m2l_targets++;
   // Processing order of the M2L targets: by level, then direction, then
@@ -1451,7 +1452,8 @@ int validatePlan(const Plan& pl, std::string& msg) {
   }
   for (int s : pl.m2fSlot)
     if (s < 0 || s >= pl.S.nslots) return bad("m2f slot");
-  if ((int)pl.m2lBeg.size() != pl.T.nslots + 1 || pl.m2lBeg.back() != (int)pl.m2lSrcF.size()) return bad("m2l csr");
+  if ((int)pl.m2lBeg.size() != pl.T.nslots + 1 || pl.m2lBeg.back() != pl.m2lEntries) return bad("m2l csr");
+  if (!pl.m2lSrcF.empty() && (int64_t)pl.m2lSrcF.size() != pl.m2lEntries) return bad("m2l entries");
   {
     // the entry checks in parallel, failures as bits
     const int64_t nm = (int64_t)pl.m2lSrcF.size();

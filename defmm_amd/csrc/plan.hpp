@@ -156,8 +156,12 @@ struct Plan {
   std::vector<TransLevel> l2l;  // top-down order
   std::vector<int> m2fSlot;     // source slots whose F is read by M2L
   std::vector<int> m2fIndex;    // per source slot: row in the F store, -1 if none
-  // M2L, CSR by target slot (every target slot appears, possibly empty)
+  // M2L, CSR by target slot (every target slot appears, possibly empty);
+  // the entries' rows are filled by the host builder only (the device
+  // builder keeps them on the device for the grouped lists, which are what
+  // the M2L kernels read)
   std::vector<int> m2lBeg;      // nslots(T)+1
+  int64_t m2lEntries = 0;       // entries of the CSR
   ListVec m2lSrcF;              // F row of the source slot
   ListVec m2lTabId;             // kernel table
   ListVec m2lHash;              // permutation
@@ -201,9 +205,10 @@ void treeFromLevels(Tree& t, const std::vector<std::vector<LevelCell>>& lv, std:
 // marks and the walk statistics, returns the sorted table keys (level,
 // fundamental offset) and the P2P cell pairs, and keeps the M2L pairs.
 // m2lLists() runs where the host builds the M2L lists: it fills m2fIndex,
-// m2fSlot and the CSR m2lBeg / m2lSrcF / m2lTabId / m2lHash, with the
-// entries of a target slot in ascending source cell order (the host walk's
-// order).  m2lGroups() builds the grouped M2L lists from them.  Each returns
+// m2fSlot, m2lBeg and m2lEntries, and keeps the entries (the host builder's
+// m2lSrcF / m2lTabId / m2lHash, a target slot's in ascending source cell
+// order, the host walk's order) on the device.  m2lGroups() builds the
+// grouped M2L lists from them.  Each returns
 // OK, an error code, or kSetupFallback when the device cannot take this plan
 // (the caller then builds it on the host).
 constexpr int kSetupFallback = -1;

@@ -942,6 +942,7 @@ int GpuSetup::m2lLists(Plan& pl, std::string& msg) {
   int64_t nm = 0;
   for (const Level& L : lv_) nm += L.nm;
   pl.m2lBeg.assign((size_t)nts + 1, 0);
+  pl.m2lEntries = 0;
   pl.m2fIndex.assign(nss, -1);
   pl.m2fSlot.clear();
   pl.m2lSrcF.clear();
@@ -1023,12 +1024,8 @@ int GpuSetup::m2lLists(Plan& pl, std::string& msg) {
     k_list_place<<<blocksFor(L.nt), 256, 0, 0>>>(a);
     SCK(hipGetLastError());
   }
-  pl.m2lSrcF.resize(nm);
-  pl.m2lTabId.resize(nm);
-  pl.m2lHash.resize(nm);
-  SCK(copyLarge(pl.m2lSrcF.data(), a.outSrcF, (size_t)nm * sizeof(int), false));
-  SCK(copyLarge(pl.m2lTabId.data(), a.outTab, (size_t)nm * sizeof(int), false));
-  SCK(copyLarge(pl.m2lHash.data(), a.outHash, (size_t)nm * sizeof(int), false));
+  // the entries stay on the device: the grouped lists are built from them
+  pl.m2lEntries = nm;
   dSrcF_ = a.outSrcF;
   dTab_ = a.outTab;
   dHash_ = a.outHash;
@@ -1054,7 +1051,7 @@ int GpuSetup::m2lGroups(Plan& pl, std::string& msg) {
     msg = "device set-up: unsupported M2L group size";
     return kSetupFallback;
   }
-  if (!pl.m2lSrcF.empty() && !dSrcF_) {
+  if (pl.m2lEntries > 0 && !dSrcF_) {
     msg = "device set-up: M2L lists not on the device";
     return kSetupFallback;
   }

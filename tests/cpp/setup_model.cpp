@@ -310,6 +310,7 @@ struct ModelSetup final : SetupDevice {
       pl.m2lTabId[q] = id;
       pl.m2lHash[q] = h;
     });
+    pl.m2lEntries = nm;
     srcF = pl.m2lSrcF;
     tab = pl.m2lTabId;
     hash = pl.m2lHash;
