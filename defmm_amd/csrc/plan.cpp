@@ -346,11 +346,13 @@ int m2lWindow(const Plan& pl, int l) {
 // numbered as the host walk numbers them; code[l] gets, per offset met,
 // table id | hash << 32 | (cone + 1) << 40 (cone -1 at LF levels), the cone
 // chosen from the representative pair as the host walk chooses it from the
-// first pair it meets with that offset (M2L.hpp:189-197).
+// first pair it meets with that offset (M2L.hpp:189-197), or taken from
+// cones (per level and offset) when the caller searched them elsewhere.
 int offsetCodes(const Plan& pl, const std::vector<int>& W,
                 const std::vector<std::vector<uint64_t>>& rep,
                 std::vector<std::vector<uint64_t>>& code,
-                std::vector<std::array<int, 4>>& tabKeys, std::string& msg) {
+                std::vector<std::array<int, 4>>& tabKeys, std::string& msg,
+                const std::vector<std::vector<int>>* cones) {
   tabKeys.clear();
   code.assign(pl.nlvl, {});
   auto offsetOf = [](int64_t k, int Wl, int iZ[3]) {
@@ -392,7 +394,9 @@ int offsetCodes(const Plan& pl, const std::vector<int>& W,
       const std::array<int, 4> key = {l, nI[0], nI[1], nI[2]};
       const int id = (int)(std::lower_bound(tabKeys.begin(), tabKeys.end(), key) - tabKeys.begin());
       int cn = -1;
-      if (hf) {
+      if (hf && cones) {
+        cn = (*cones)[l][k];
+      } else if (hf) {
         const int i = (int)(r >> 32), j = (int)(r & 0xFFFFFFFFu);
         double fZ[3];
         for (int q = 0; q < 3; q++) fZ[q] = pl.T.ctr[3 * i + q] - pl.S.ctr[3 * j + q];

@@ -241,11 +241,14 @@ int buildPlan(Plan& pl, const double* src, int64_t ns, const double* trg, int64_
 int m2lWindow(const Plan& pl, int l);
 
 // Table keys and per-offset codes (table id | hash << 32 | (cone + 1) << 40)
-// from the representative pair of each offset met at each level.
+// from the representative pair of each offset met at each level; the cones
+// of the HF levels are searched here unless cones (per level and offset,
+// found with bestCone's operations elsewhere) is given.
 int offsetCodes(const Plan& pl, const std::vector<int>& W,
                 const std::vector<std::vector<uint64_t>>& rep,
                 std::vector<std::vector<uint64_t>>& code,
-                std::vector<std::array<int, 4>>& tabKeys, std::string& msg);
+                std::vector<std::array<int, 4>>& tabKeys, std::string& msg,
+                const std::vector<std::vector<int>>* cones = nullptr);
 
 // The device set-up (setup.hip): nullptr when no device is usable.
 SetupDevice* newSetupDevice();

@@ -28,8 +28,9 @@
 // The M2L pairs stay on the device.  Each offset iZ = idx(target) -
 // idx(source) a level meets gets a representative pair (the smallest
 // (target, source), by a 64-bit atomic minimum over a dense window of
-// offsets); the host turns the representatives into table ids, permutation
-// hashes and cones (offsetCodes, plan.cpp), and the device then marks the
+// offsets); the device searches each offset's cone from it (k_cones), the
+// host turns the representatives into table ids and permutation hashes
+// (offsetCodes, plan.cpp), and the device then marks the
 // effective directions of both cells of every HF pair.  After the host has
 // numbered the slots, the device builds the M2L CSR by target slot: a count
 // per slot (atomics), the host's prefix sum, then a stable placement in
@@ -262,6 +263,52 @@ __global__ __launch_bounds__(256) void k_tree_part(const TreeChunk* __restrict__
     const int o = code[c.start + i];
     out[pos[o]++] = perm[c.start + i];
   }
+}
+
+// Cone of each offset met (M2L.hpp:189-197, the host's bestCone): the
+// direction nearest to the representative pair's normalised centre
+// difference, the first within 1e-12 of the best winning, with the host's
+// operations in the host's order (no contraction in this file; double
+// square root and division correctly rounded).
+__global__ __launch_bounds__(256) void k_cones(const unsigned long long* __restrict__ rep, int n,
+                                               const double* __restrict__ Tctr,
+                                               const double* __restrict__ Sctr,
+                                               const double* __restrict__ dirs, int ndir,
+                                               int* __restrict__ cone) {
+#pragma clang fp contract(off)
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const unsigned long long r = rep[k];
+  if (r == ~0ull) {
+    cone[k] = -1;
+    return;
+  }
+  const int i = (int)(r >> 32), j = (int)(r & 0xFFFFFFFFull);
+  double f0 = Tctr[3 * i] - Sctr[3 * j], f1 = Tctr[3 * i + 1] - Sctr[3 * j + 1],
+         f2 = Tctr[3 * i + 2] - Sctr[3 * j + 2];
+  double t = 0.;
+  t += f0 * f0;
+  t += f1 * f1;
+  t += f2 * f2;
+  const double nn = sqrt(t);
+  f0 /= nn;
+  f1 /= nn;
+  f2 /= nn;
+  double best = 1.e30;
+  int ind = -1;
+  for (int d = 0; d < ndir; d++) {
+    const double e0 = dirs[3 * d] - f0, e1 = dirs[3 * d + 1] - f1, e2 = dirs[3 * d + 2] - f2;
+    double u = 0.;
+    u += e0 * e0;
+    u += e1 * e1;
+    u += e2 * e2;
+    const double m = sqrt(u);
+    if (m < best - 1.e-12) {
+      best = m;
+      ind = d;
+    }
+  }
+  cone[k] = ind;
 }
 
 // The M2L pairs of one level, CSR by target cell (the level's target list).
@@ -892,7 +939,27 @@ int GpuSetup::walk(Plan& pl, std::vector<std::array<int, 4>>& tabKeys, std::vect
     hrep[l].resize(n);
     SCK(hipMemcpy(hrep[l].data(), rep[l], n * sizeof(uint64_t), hipMemcpyDeviceToHost));
   }
-  int r = offsetCodes(pl, W, hrep, code, tabKeys, msg);
+  // cones of the HF levels' offsets on the device
+  std::vector<std::vector<int>> cones(pl.nlvl);
+  for (int l = 0; l < pl.nlvl; l++) {
+    if (!rep[l] || !(l && pl.isHF[l])) continue;
+    const std::vector<std::array<double, 3>>& D = pl.dirs[pl.HFStage[l]];
+    std::vector<double> dd(3 * D.size());
+    for (size_t d = 0; d < D.size(); d++)
+      for (int q = 0; q < 3; q++) dd[3 * d + q] = D[d][q];
+    const int n = (int)hrep[l].size();
+    double* dD = nullptr;
+    int* dc = nullptr;
+    SCK(put(dD, dd));
+    SCK(alloc(dc, (size_t)n));
+    k_cones<<<(unsigned)((n + 255) / 256), 256, 0, 0>>>(rep[l], n, Tctr, Sctr, dD, (int)D.size(), dc);
+    SCK(hipGetLastError());
+    cones[l].resize(n);
+    SCK(hipMemcpy(cones[l].data(), dc, (size_t)n * sizeof(int), hipMemcpyDeviceToHost));
+    release(dD);
+    release(dc);
+  }
+  int r = offsetCodes(pl, W, hrep, code, tabKeys, msg, &cones);
   if (r != OK) return r;
   for (int l = 0; l < pl.nlvl; l++) {
     if (!rep[l]) continue;
