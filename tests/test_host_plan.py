@@ -118,3 +118,17 @@ def test_degenerate_inputs():
         dm.build_plan_stats(pts, pts, 4, 8, float("nan"))
     with pytest.raises(dm.DefmmError, match="no M2L level"):
         dm.build_plan_stats(pts[:2], pts[:2], 4, 8, 1.0)
+
+
+def test_setup_mode_switch():
+    """defmm_set_setup_mode (no device work): returns the previous mode and
+    refuses an unknown one, leaving the mode unchanged."""
+    dm, _ = _lib_or_skip()
+    old = dm.set_setup_mode(dm.SETUP_HOST)
+    try:
+        assert dm.set_setup_mode(dm.SETUP_DEVICE) == dm.SETUP_HOST
+        with pytest.raises(dm.DefmmError, match="unknown set-up mode"):
+            dm.set_setup_mode(7)
+        assert dm.set_setup_mode(dm.SETUP_DEVICE) == dm.SETUP_DEVICE
+    finally:
+        dm.set_setup_mode(old)
