@@ -27,8 +27,10 @@ one after another, then all of them by one HIP graph (defmm_batch_*).
 
 Scale section (--scale-workload, default the 1M-point sphere at 128
 wavelengths, order 5): FP64 and mixed at full size; at this frequency order 5
-gives 7e-3, and the order that reaches 1e-6 does not fit in one GPU's memory
-(see the design notes), so it reports throughput and error as measured.
+gives 7e-3, and the order that reaches 1e-6 there (about 10) does not fit in
+one GPU's memory, so it reports throughput and error as measured.  It
+also times the same plan's set-up by the host builder alone (host-only
+build, no upload) against the default device set-up without its upload.
 
 Roofline: the dominant stage of the value pass, the fused M2L+F2L (with, in
 the batched M2L variant, the M2F launches that refill its workspace).
@@ -545,6 +547,20 @@ def main():
                            "speedup_vs_fp64": s64 / smx, "autotune": stunes.get("mixed")},
                  "note": "order 5 at 128 wavelengths: error 7e-3, above the 1e-6 gate"}
         S.close()
+        # the same plan built by the host builder alone (no device work, no
+        # upload), against the device set-up without its upload
+        t0 = time.time()
+        hs = dm.build_plan_stats(S.pts, S.pts, S.L, S.ncrit, S.kappa)
+        host_s = time.time() - t0
+        dev_s = S.setup_s - S.setup_ms.get("upload", 0) / 1e3
+        scale["setup_host_builder_s"] = host_s
+        scale["setup_device_builder_s_without_upload"] = dev_s
+        scale["setup_plans_equal"] = all(hs[k] == S.st[k] for k in hs
+                                         if not k.startswith("ms_") and k not in
+                                         ("setup_device", "m2l_batches", "m2l_batch_rows",
+                                          "m2l_table_rows", "p2p_items"))
+        log(f"set-up without upload: device builder {dev_s:.2f} s, host builder {host_s:.2f} s, "
+            f"same lists {scale['setup_plans_equal']}")
 
     many = None
     if args.multi_rhs_workload != "none":

@@ -4,8 +4,8 @@ phase tables overflow there (include/calculus/planewave.hpp:44-55 index arrays
 of "HF levels >= 2" by HF stage), so no reference-held vector covers it; what
 fixes the answer is the mathematics: the error of equispaced interpolation
 of order L against the FP64 direct sum must fall geometrically with L (about
-one decade per order, SURVEY measurements on the reference: 6.3e-3, 1.1e-3,
-1.0e-4, 1.1e-5, 9.2e-7 at L = 3..7).  A wrong phase row, direction or table
+one decade per order; measurements of the reference itself in its own
+regime gave 6.3e-3, 1.1e-3, 1.0e-4, 1.1e-5, 9.2e-7 at L = 3..7).  A wrong phase row, direction or table
 in the all-HF levels stops that decay at the first level it touches.
 
 CPU only: four oracle evaluations (L = 4..7) run side by side in threads
